@@ -1,0 +1,167 @@
+// chol_abi.cpp — the extern "C" boundary (include/chol_amd.h): argument
+// validation, workspace sizing and dispatch to the gfx950 kernels.
+//
+// Dispatch by order n (the reference dispatches by (batch, n) tables in
+// custom_kernel, submission.py:4101-4268; here the route depends only on n so
+// every (batch, n) takes a HIP route and there is no library fallback):
+//   n <= 64 : chol_small.hip, whole matrix in one wavefront's registers;
+//   n >  64 : chol_blocked.hip, 64-wide diagonal blocks with explicit
+//             inverses, panel GEMM and lower-tile SYRK.
+#include <cstdarg>
+#include <cstring>
+
+#include "chol_common.h"
+
+namespace chol {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+void clear_error() { g_err[0] = '\0'; }
+
+// Outer block width of the blocked path (columns per trailing SYRK).
+static int outer_block(int64_t n) { return n <= 256 ? 256 : 256; }
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static size_t potrf_ws(int64_t batch, int64_t n) {
+  if (n <= 64 || batch <= 0) return 0;
+  return align256((size_t)batch * kNB * kNB * sizeof(float));
+}
+
+static int blocked_potrf(const Batch& p, float* winv, hipStream_t s) {
+  int rc = launch_copy_lower(p, s);
+  if (rc) return rc;
+  const int n = p.n;
+  const int nbo = outer_block(n);
+  for (int K0 = 0; K0 < n; K0 += nbo) {
+    const int W = (n - K0) < nbo ? (n - K0) : nbo;
+    for (int k0 = K0; k0 < K0 + W; k0 += kNB) {
+      if ((rc = launch_diag_factor(p, k0, winv, s))) return rc;
+      if (k0 + kNB >= n) break;
+      if ((rc = launch_panel(p, k0, winv, s))) return rc;
+      if (k0 + kNB < K0 + W) {
+        if ((rc = launch_syrk(p, k0 + kNB, k0 + kNB, K0 + W, k0, kNB, s))) return rc;
+      }
+    }
+    if (K0 + W < n) {
+      if ((rc = launch_syrk(p, K0 + W, K0 + W, n, K0, W, s))) return rc;
+    }
+  }
+  return 0;
+}
+
+}  // namespace chol
+
+using namespace chol;
+
+extern "C" {
+
+int chol_abi_version(void) { return CHOL_ABI_VERSION; }
+
+const char* chol_last_error(void) { return g_err; }
+
+size_t chol_workspace_bytes(int64_t batch, int64_t n, int32_t mode) {
+  (void)mode;
+  return potrf_ws(batch, n);
+}
+
+int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n, int64_t batch_stride, int64_t ld,
+                   int32_t mode, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
+  clear_error();
+  if (batch < 0 || n < 0) {
+    set_error("chol_potrf_f32: batch (%lld) and n (%lld) must be >= 0", (long long)batch, (long long)n);
+    return CHOL_E_ARG;
+  }
+  if (mode != CHOL_MODE_PLAIN && mode != CHOL_MODE_MIXED) {
+    set_error("chol_potrf_f32: unknown mode %d", (int)mode);
+    return CHOL_E_MODE;
+  }
+  if (batch == 0 || n == 0) return 0;
+  if (n > 65536 || batch > (int64_t)1 << 31) {
+    set_error("chol_potrf_f32: n=%lld or batch=%lld out of range", (long long)n, (long long)batch);
+    return CHOL_E_ARG;
+  }
+  if (!A || !L) {
+    set_error("chol_potrf_f32: null A or L");
+    return CHOL_E_ARG;
+  }
+  if (ld < n || (batch > 1 && batch_stride < n * ld)) {
+    set_error("chol_potrf_f32: ld=%lld must be >= n=%lld and batch_stride=%lld >= n*ld", (long long)ld,
+              (long long)n, (long long)batch_stride);
+    return CHOL_E_ARG;
+  }
+  if (mode == CHOL_MODE_MIXED) {
+    set_error("chol_potrf_f32: mixed mode is not available in this build");
+    return CHOL_E_MODE;
+  }
+  const size_t need = potrf_ws(batch, n);
+  if (need > 0 && (ws == nullptr || ws_bytes < need)) {
+    set_error("chol_potrf_f32: workspace of %zu bytes required, got %zu", need, ws_bytes);
+    return CHOL_E_WORKSPACE;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Batch p{A, L, batch, (int)n, batch > 1 ? batch_stride : n * ld, ld, info};
+  if (n <= 64) return launch_small(p, s);
+  int rc = launch_zero_info(info, batch, s);
+  if (rc) return rc;
+  // The blocked kernels put the batch on grid.y (at most 65535 blocks).
+  constexpr int64_t kMaxY = 65535;
+  for (int64_t b0 = 0; b0 < batch; b0 += kMaxY) {
+    Batch q = p;
+    q.batch = (batch - b0) < kMaxY ? (batch - b0) : kMaxY;
+    q.A = A + b0 * p.bs;
+    q.L = L + b0 * p.bs;
+    q.info = info ? info + b0 : nullptr;
+    if ((rc = blocked_potrf(q, reinterpret_cast<float*>(ws), s))) return rc;
+  }
+  return 0;
+}
+
+size_t chol_ragged_workspace_bytes(const int64_t* n_host, int64_t count) {
+  (void)n_host;
+  (void)count;
+  return 0;
+}
+
+int chol_potrf_ragged_f32(const float* const* A, float* const* L, const int64_t* n_dev, const int64_t* n_host,
+                          int64_t count, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
+  (void)A; (void)L; (void)n_dev; (void)n_host; (void)count; (void)info; (void)ws; (void)ws_bytes; (void)stream;
+  set_error("chol_potrf_ragged_f32: not available in this build");
+  return CHOL_E_ARG;
+}
+
+size_t chol_backward_workspace_bytes(int64_t batch, int64_t n) {
+  (void)batch;
+  (void)n;
+  return 0;
+}
+
+int chol_backward_f32(const float* Lm, const float* Lbar, float* Abar, int64_t batch, int64_t n, const float* Linv,
+                      int32_t have_linv, void* ws, size_t ws_bytes, void* stream) {
+  (void)Lm; (void)Lbar; (void)Abar; (void)batch; (void)n; (void)Linv; (void)have_linv; (void)ws; (void)ws_bytes;
+  (void)stream;
+  set_error("chol_backward_f32: not available in this build");
+  return CHOL_E_ARG;
+}
+
+size_t chol_tri_inverse_workspace_bytes(int64_t batch, int64_t n) {
+  (void)batch;
+  (void)n;
+  return 0;
+}
+
+int chol_tri_inverse_f32(const float* Lm, float* Linv, int64_t batch, int64_t n, void* ws, size_t ws_bytes,
+                         void* stream) {
+  (void)Lm; (void)Linv; (void)batch; (void)n; (void)ws; (void)ws_bytes; (void)stream;
+  set_error("chol_tri_inverse_f32: not available in this build");
+  return CHOL_E_ARG;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+/*
+ * chol_amd.h — C ABI of the MI355X (gfx950) batched Cholesky engine.
+ *
+ * Every entry point takes plain device pointers, sizes and strides, and the
+ * HIP stream to launch on (passed as void*; a hipStream_t, 0 = null stream).
+ * No torch or HIP types appear in the signatures, so ctypes, cgo or any other
+ * FFI can bind them.  Nothing here synchronises the stream, allocates device
+ * memory on the hot path, or frees caller memory.
+ *
+ * Layout: a batch of `batch` matrices, matrix b starting at
+ * A + b*batch_stride, row-major with leading dimension `ld` (elements), so
+ * A[b][i][j] = A[b*batch_stride + i*ld + j].  Only the lower triangle of A is
+ * read.  L receives the lower-triangular factor with exact zeros above the
+ * diagonal.  A is never written; A and L must not overlap.
+ *
+ * Return value: 0 on success, a negative CHOL_E* code on a rejected argument
+ * (nothing is launched then), or a positive hipError_t from a launch.
+ * chol_last_error() gives a message for the last non-zero return on the
+ * calling thread.
+ *
+ * Non-SPD input is not an error: like LAPACK potrf with info, the factor then
+ * holds NaN from the first non-positive pivot on, and info[b] (when info is
+ * non-null) is set to the 1-based order of the first leading minor that is
+ * not positive definite, 0 when the matrix factored.
+ */
+#ifndef CHOL_AMD_H
+#define CHOL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHOL_ABI_VERSION 1
+
+/* Error codes (negative; positive values are hipError_t). */
+#define CHOL_E_ARG (-1)       /* bad size, stride or null pointer            */
+#define CHOL_E_WORKSPACE (-2) /* workspace missing or smaller than required  */
+#define CHOL_E_MODE (-3)      /* unknown precision mode                      */
+
+/* Precision modes. */
+#define CHOL_MODE_PLAIN 0 /* all-FP32 arithmetic                            */
+#define CHOL_MODE_MIXED 1 /* split-BF16 trailing updates + FP64 correction  */
+
+/* Version of this ABI (CHOL_ABI_VERSION). */
+int chol_abi_version(void);
+
+/* Message for the last non-zero return of any chol_* call on this thread. */
+const char* chol_last_error(void);
+
+/*
+ * Device workspace (bytes) that chol_potrf_f32 needs for this problem.  The
+ * caller allocates it (any 256-byte aligned device buffer) and passes it in.
+ * Replaces: the module-global cached workspaces of the reference dispatcher
+ * (submission.py:5134-5149 `_exp062_buffers`, :2569-2570 split32 buffers).
+ */
+size_t chol_workspace_bytes(int64_t batch, int64_t n, int32_t mode);
+
+/*
+ * Batched Cholesky factorisation A = L L^T, FP32 in and out.
+ * Replaces: custom_kernel(data) -> L (submission.py:4101-4268) together with
+ * the pybind launchers it reaches: chol32_launch (submission.py:168),
+ * chol64_launch (:275), chol128_launch (:566), micro32_launch (:2412),
+ * chol256_launch (:1091), e62_diag128_launch / e62_zero_upper_launch
+ * (:5032-5101), and the torch.linalg.cholesky_ex fallback (:4268).
+ *
+ *   A, L          device pointers (see Layout above)
+ *   batch, n      batch >= 0, n >= 0 (batch == 0 or n == 0 launches nothing)
+ *   batch_stride  >= n*ld when batch > 1
+ *   ld            >= max(n, 1)
+ *   mode          CHOL_MODE_PLAIN or CHOL_MODE_MIXED
+ *   info          nullable device int32[batch]
+ *   ws, ws_bytes  device workspace of at least chol_workspace_bytes()
+ *   stream        hipStream_t to launch on
+ */
+int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n,
+                   int64_t batch_stride, int64_t ld, int32_t mode,
+                   int32_t* info, void* ws, size_t ws_bytes, void* stream);
+
+/*
+ * Ragged batch: `count` independent matrices of mixed orders n[i], each
+ * contiguous (ld = n[i]).  The pointer and size arrays live in DEVICE memory;
+ * n_host is the same sizes in host memory (used to size the launches).
+ * Has no counterpart in the reference (which factors one fixed (batch, n) per
+ * call); it generalises the per-shape dispatch of submission.py:4101-4268.
+ */
+size_t chol_ragged_workspace_bytes(const int64_t* n_host, int64_t count);
+int chol_potrf_ragged_f32(const float* const* A, float* const* L,
+                          const int64_t* n_dev, const int64_t* n_host,
+                          int64_t count, int32_t* info, void* ws,
+                          size_t ws_bytes, void* stream);
+
+/*
+ * Adjoint of the factorisation: given L (from chol_potrf_f32) and the
+ * upstream gradient Lbar (lower triangle read), writes the symmetric
+ *   Abar = 1/2 * sym(L^-T Phi(L^T Lbar) L^-1),
+ * Phi(X) = lower triangle of X with its diagonal halved, sym(X) = X + X^T.
+ * This is the gradient torch's autograd gives for torch.linalg.cholesky
+ * (full symmetric matrix, not the lower-triangle-only gradient).
+ * Linv: nullable device buffer batch x n x n (ld = n).  When given with
+ * have_linv = 1, it must hold L^-1 (e.g. from chol_tri_inverse_f32) and it is
+ * used as is; when null, L^-1 is recomputed inside ws.
+ * No counterpart in the reference (it has no backward); replaces the torch
+ * autograd formula of torch.linalg.cholesky for callers that differentiate.
+ */
+size_t chol_backward_workspace_bytes(int64_t batch, int64_t n);
+int chol_backward_f32(const float* L, const float* Lbar, float* Abar,
+                      int64_t batch, int64_t n, const float* Linv,
+                      int32_t have_linv, void* ws, size_t ws_bytes,
+                      void* stream);
+
+/* Lower-triangular inverse Linv = L^-1 (batch x n x n, ld = n, contiguous). */
+size_t chol_tri_inverse_workspace_bytes(int64_t batch, int64_t n);
+int chol_tri_inverse_f32(const float* L, float* Linv, int64_t batch,
+                         int64_t n, void* ws, size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHOL_AMD_H */

@@ -1,0 +1,91 @@
+"""The C ABI without a GPU: the library loads, exports every symbol the
+header declares, and rejects bad arguments before any launch."""
+import ctypes
+import os
+import re
+
+import pytest
+import torch
+
+from gpumode_cholesky_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "chol_amd.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(chol_\w+)\s*\(", text)))
+
+
+def test_header_symbols_exported():
+    lib = _lib.load()
+    names = _declared()
+    assert "chol_potrf_f32" in names
+    for name in names:
+        assert hasattr(lib, name), f"{name} declared but not exported"
+    # and the Python binding knows every declared symbol
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_abi_version():
+    assert _lib.load().chol_abi_version() == 1
+
+
+FAKE = ctypes.c_void_p(0x1000)  # never dereferenced: every call below fails validation
+
+
+def _potrf(**kw):
+    args = dict(A=FAKE, L=ctypes.c_void_p(0x2000), batch=1, n=8, bs=64, ld=8, mode=0,
+                info=ctypes.c_void_p(0), ws=ctypes.c_void_p(0), wsb=0, stream=ctypes.c_void_p(0))
+    args.update(kw)
+    return _lib.load().chol_potrf_f32(args["A"], args["L"], args["batch"], args["n"], args["bs"],
+                                      args["ld"], args["mode"], args["info"], args["ws"], args["wsb"],
+                                      args["stream"])
+
+
+def test_rejects_negative_sizes():
+    assert _potrf(batch=-1) == _lib.E_ARG
+    assert "batch" in _lib.last_error()
+    assert _potrf(n=-3) == _lib.E_ARG
+
+
+def test_empty_problem_is_a_no_op():
+    assert _potrf(batch=0) == 0
+    assert _potrf(n=0) == 0
+
+
+def test_rejects_bad_mode_and_strides():
+    assert _potrf(mode=7) == _lib.E_MODE
+    assert _potrf(ld=4) == _lib.E_ARG
+    assert _potrf(batch=2, bs=10) == _lib.E_ARG
+    assert _potrf(A=ctypes.c_void_p(0)) == _lib.E_ARG
+
+
+def test_rejects_short_workspace():
+    lib = _lib.load()
+    need = lib.chol_workspace_bytes(4, 200, 0)
+    assert need >= 4 * 64 * 64 * 4
+    assert _potrf(batch=4, n=200, ld=200, bs=200 * 200) == _lib.E_WORKSPACE
+    assert _potrf(batch=4, n=200, ld=200, bs=200 * 200, ws=ctypes.c_void_p(0x3000),
+                  wsb=need - 1) == _lib.E_WORKSPACE
+    assert "workspace" in _lib.last_error()
+    assert lib.chol_workspace_bytes(4096, 32, 0) == 0  # wave-register path needs none
+
+
+def test_custom_kernel_refuses_cpu_tensors():
+    from gpumode_cholesky_amd import custom_kernel
+    a = torch.eye(4).unsqueeze(0)
+    with pytest.raises(ValueError, match="HIP device"):
+        custom_kernel(a)
+
+
+def test_custom_kernel_refuses_bad_dtype_and_shape():
+    from gpumode_cholesky_amd import cholesky
+    with pytest.raises(TypeError):
+        cholesky(torch.eye(4, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        cholesky(torch.zeros(2, 3, 4), mode="plain")
+    with pytest.raises(ValueError):
+        cholesky(torch.eye(3), mode="fp16")

@@ -1,0 +1,175 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 CPU oracle
+and against torch.linalg.cholesky on the same device.
+
+Tolerances (FP32 arithmetic, stated here and used below):
+  * reconstruction: ||L L^T - A||_F / ||A||_F (FP64) <= RESID_C * sqrt(n) * eps32
+  * forward error:  ||L - L64||_max / ||L64||_max <= FWD_C * e_torch + FWD_ABS,
+    where e_torch is the same error of torch.linalg.cholesky on the same GPU
+    (the reference's own ref_kernel, reference/reference.py:129-130) and L64
+    the FP64 oracle factor;
+  * the reference's gate (reference/reference.py:137-188) must pass;
+  * exact zeros above the diagonal, positive diagonal, input unchanged.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from gpumode_cholesky_amd import cholesky, custom_kernel
+from gpumode_cholesky_amd import harness
+
+pytestmark = pytest.mark.gpu
+
+EPS32 = float(np.finfo(np.float32).eps)
+RESID_C = 8.0
+FWD_C = 20.0
+FWD_ABS = 8 * EPS32
+DEV = torch.device("cuda")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _check_against_oracle(oracle_mod, a_cpu, label):
+    a = a_cpu.to(DEV)
+    a_copy = a.clone()
+    L = custom_kernel(a)
+    torch.cuda.synchronize()
+    assert torch.equal(a, a_copy), "input was modified"
+    ok, msg = harness.check_implementation(a, L)
+    assert ok, f"{label}: {msg}"
+    Lc = L.cpu().numpy()
+    n = a.shape[-1]
+    assert (np.triu(Lc, 1) == 0).all(), "non-zero above the diagonal"
+    L64, info = oracle_mod.potrf64(a_cpu.numpy())
+    assert (info == 0).all()
+    fro, _, _ = oracle_mod.residual(a_cpu.numpy(), Lc)
+    Lt = torch.linalg.cholesky(a).cpu().numpy().astype(np.float64)
+    scale = np.abs(L64).reshape(L64.shape[0], -1).max(axis=1)
+    e_ours = (np.abs(Lc - L64).reshape(L64.shape[0], -1).max(axis=1) / scale).max()
+    e_torch = (np.abs(Lt - L64).reshape(L64.shape[0], -1).max(axis=1) / scale).max()
+    _say(f"[parity] {label}: resid={fro.max():.3e} fwd={e_ours:.3e} torch_fwd={e_torch:.3e} {msg}")
+    assert fro.max() <= RESID_C * math.sqrt(n) * EPS32, f"{label}: residual {fro.max():.3e}"
+    assert e_ours <= FWD_C * e_torch + FWD_ABS, f"{label}: forward error {e_ours:.3e} vs torch {e_torch:.3e}"
+    return L
+
+
+SMALL_N = [1, 2, 3, 5, 8, 16, 17, 31, 32, 33, 48, 63, 64]
+
+
+@pytest.mark.parametrize("case,cond", harness.FAMILIES)
+@pytest.mark.parametrize("n", SMALL_N)
+def test_small_families(oracle_mod, n, case, cond):
+    a = harness.generate_input(7, n, cond, 1000 + n, case, device="cpu")
+    _check_against_oracle(oracle_mod, a, f"small n={n} {case}")
+
+
+BLOCKED_N = [65, 96, 127, 128, 129, 200, 256, 300, 320, 512, 700]
+
+
+@pytest.mark.parametrize("case,cond", [("dense", 2), ("spectrum", 5), ("rowscale", 4), ("tridiagonal", 1)])
+@pytest.mark.parametrize("n", BLOCKED_N)
+def test_blocked_families(oracle_mod, n, case, cond):
+    a = harness.generate_input(3, n, cond, 2000 + n, case, device="cpu")
+    _check_against_oracle(oracle_mod, a, f"blocked n={n} {case}")
+
+
+@pytest.mark.parametrize("batch,n", [(4096, 32), (1024, 64), (256, 128), (64, 256), (16, 512), (4, 1024)])
+def test_config_shapes_against_oracle(oracle_mod, batch, n):
+    # The benchmark workload generator at its real shape; the oracle checks a
+    # sampled subset of the batch (the whole batch for the smaller n).
+    a = harness.config_input(batch, n, 77 + n, device="cpu")
+    L = custom_kernel(a.to(DEV)).cpu()
+    idx = np.linspace(0, batch - 1, min(batch, 16)).astype(int)
+    sub = a[idx]
+    L64, _ = oracle_mod.potrf64(sub.numpy())
+    fro, _, _ = oracle_mod.residual(sub.numpy(), L[idx].numpy())
+    e = np.abs(L[idx].numpy() - L64).max() / np.abs(L64).max()
+    _say(f"[config] {batch}x{n}: resid={fro.max():.3e} fwd={e:.3e}")
+    assert fro.max() <= RESID_C * math.sqrt(n) * EPS32
+    ok, msg = harness.check_implementation(a.to(DEV), L.to(DEV))
+    assert ok, msg
+
+
+@pytest.mark.parametrize("spec", harness.BENCH_SPECS, ids=lambda s: f"{s['batch']}x{s['n']}")
+def test_bench_shapes_gate(spec):
+    # Full benchmark shapes, inputs generated on the GPU: the reference gate,
+    # the FP64 residual and agreement with torch.linalg.cholesky.
+    b, n = spec["batch"], spec["n"]
+    _say(f"[bench-shape] start {b}x{n}")
+    a = harness.config_input(b, n, spec["seed"], device=DEV)
+    L = custom_kernel(a)
+    torch.cuda.synchronize()
+    ok, msg = harness.check_implementation(a, L)
+    assert ok, f"{b}x{n}: {msg}"
+    resid = harness.fp64_residual(a, L)
+    line = f"[bench-shape] {b}x{n}: {msg} fp64_resid={resid:.3e}"
+    if n <= 4096:
+        Lt = torch.linalg.cholesky(a)
+        d = ((L - Lt).abs().amax() / Lt.abs().amax()).item()
+        line += f" vs_torch={d:.3e}"
+        assert d < 1e-3
+        del Lt
+    _say(line)
+    assert resid <= RESID_C * math.sqrt(n) * EPS32
+    assert torch.count_nonzero(torch.triu(L, 1)).item() == 0
+    del a, L
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n", [7, 32, 64, 100, 300])
+def test_non_spd_sets_info_and_nan(n):
+    a = harness.generate_input(3, n, 2, 5, device="cpu")
+    k = n // 2
+    a[1, k, k] = -1.0  # the leading minor of order k+1 is indefinite
+    L, info = cholesky(a.to(DEV), return_info=True)
+    info = info.cpu()
+    assert info[0].item() == 0 and info[2].item() == 0
+    assert info[1].item() == k + 1
+    Lc = L.cpu()
+    assert torch.isfinite(Lc[0]).all() and torch.isfinite(Lc[2]).all()
+    assert torch.isnan(Lc[1, k, k])
+
+
+@pytest.mark.parametrize("n", [24, 64, 160])
+def test_strided_and_transposed_inputs(oracle_mod, n):
+    base = harness.generate_input(4, n, 2, 9, device="cpu").to(DEV)
+    ref = custom_kernel(base.contiguous())
+    # non-contiguous batch (every other matrix of a larger buffer)
+    big = torch.zeros((8, n, n), device=DEV)
+    big[::2] = base
+    out = custom_kernel(big[::2])
+    assert torch.equal(out, ref)
+    # transposed view of a symmetric matrix: same lower triangle
+    out_t = custom_kernel(base.transpose(-1, -2))
+    assert torch.equal(out_t, ref)
+    # 2-D input
+    out2 = custom_kernel(base[1])
+    assert out2.shape == (n, n) and torch.equal(out2, ref[1])
+
+
+@pytest.mark.parametrize("n", [32, 64, 128, 512])
+def test_upper_triangle_of_input_is_ignored(n):
+    a = harness.generate_input(2, n, 2, 3, device="cpu").to(DEV)
+    garbage = a.clone()
+    iu = torch.triu_indices(n, n, 1, device=DEV)
+    garbage[:, iu[0], iu[1]] = float("nan")
+    assert torch.equal(custom_kernel(garbage), custom_kernel(a))
+
+
+@pytest.mark.parametrize("batch,n", [(4096, 32), (1024, 64), (64, 256), (4, 1024)])
+def test_bitwise_determinism(batch, n):
+    a = harness.config_input(batch, n, 5, device=DEV)
+    outs = [custom_kernel(a) for _ in range(3)]
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[1], outs[2])
+
+
+def test_empty_batch_and_n1():
+    a = torch.empty((0, 5, 5), device=DEV)
+    assert custom_kernel(a).shape == (0, 5, 5)
+    a = torch.full((3, 1, 1), 9.0, device=DEV)
+    assert torch.equal(custom_kernel(a), torch.full((3, 1, 1), 3.0, device=DEV))
