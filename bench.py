@@ -12,7 +12,11 @@ of cache and times, with HIP events, one call per rotated input
 per-call mean over the K steps is that shape's time.  The metric is the
 geometric mean of the 15 per-shape means (microseconds, lower is better).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|mixed|both]
+
+Both forms are timed by default; the headline value is the mixed form
+(BF16x3 trailing updates), reported with its FP64 residual beside the plain
+all-FP32 form's.
 """
 from __future__ import annotations
 
@@ -37,7 +41,7 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="plain", choices=["plain", "mixed"])
+    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "both"])
     ap.add_argument("--shapes", default="", help="comma list like 4096x32,1x4096 (default: all 15)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
@@ -102,46 +106,45 @@ def cpu_baseline(budget_s: float):
     }
 
 
-def main():
-    args = _parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
-                          "(a multi-GPU run is not part of this build)"}))
-        sys.exit(2)
-    dev = torch.device("cuda")
-    torch.cuda.set_device(dev)
-    specs = _specs(args.shapes)
-    rows = []
-    t_all = time.perf_counter()
+def _inputs(specs, dev):
     data = {}
-    # Inputs + correctness gate for every shape (untimed).
     for spec in specs:
         b, n = spec["batch"], spec["n"]
-        cnt = harness.rotation_count(b, n)
-        inputs = []
-        for i in range(cnt):
-            inputs.append(harness.config_input(b, n, spec["seed"] + 42 * i, device=dev))
-        L = cholesky(inputs[0], mode=args.mode)
-        torch.cuda.synchronize()
-        ok, msg = harness.check_implementation(inputs[0], L)
-        resid = harness.fp64_residual(inputs[0], L)
-        if not ok:
-            print(json.dumps({"error": f"{b}x{n}: {msg}"}))
-            sys.exit(1)
-        del L
-        data[(b, n)] = inputs
-        rows.append({"batch": b, "n": n, "rotation": cnt, "check": msg, "fp64_resid": resid})
-        print(f"# ready {b}x{n}: {msg} fp64_resid={resid:.2e}", file=sys.stderr, flush=True)
+        data[(b, n)] = [harness.config_input(b, n, spec["seed"] + 42 * i, device=dev)
+                        for i in range(harness.rotation_count(b, n))]
+        print(f"# inputs {b}x{n} x{len(data[(b, n)])}", file=sys.stderr, flush=True)
+    return data
 
-    fn = (lambda d: cholesky(d, mode=args.mode))
-    for _ in range(args.warmup):
+
+def _gate(specs, data, mode):
+    """Correctness of one form on every shape (untimed): the reference's gate
+    and the FP64 residual ||LL^T - A||_F / ||A||_F."""
+    out = {}
+    for spec in specs:
+        key = (spec["batch"], spec["n"])
+        a = data[key][0]
+        L = cholesky(a, mode=mode)
+        torch.cuda.synchronize()
+        ok, msg = harness.check_implementation(a, L)
+        if not ok:
+            print(json.dumps({"error": f"{mode} {key[0]}x{key[1]}: {msg}"}))
+            sys.exit(1)
+        out[key] = harness.fp64_residual(a, L)
+        del L
+        print(f"# gate {mode} {key[0]}x{key[1]}: {msg} fp64_resid={out[key]:.2e}",
+              file=sys.stderr, flush=True)
+    return out
+
+
+def _time(specs, data, mode, steps, warmup, dev):
+    fn = (lambda d: cholesky(d, mode=mode))
+    for _ in range(warmup):
         for spec in specs:
             for d in data[(spec["batch"], spec["n"])]:
                 fn(d)
     torch.cuda.synchronize()
-    per_shape = {(s["batch"], s["n"]): [] for s in specs}
-    t0 = time.perf_counter()
-    for step in range(args.steps):
+    ev = {(s["batch"], s["n"]): [] for s in specs}
+    for step in range(steps):
         for spec in specs:
             key = (spec["batch"], spec["n"])
             harness.flush_caches(dev)
@@ -151,40 +154,71 @@ def main():
             for d in data[key]:
                 fn(d)
             e1.record()
-            per_shape[key].append((e0, e1, len(data[key])))
+            ev[key].append((e0, e1, len(data[key])))
         torch.cuda.synchronize()
-        print(f"# step {step + 1}/{args.steps} done", file=sys.stderr, flush=True)
+        print(f"# {mode} step {step + 1}/{steps} done", file=sys.stderr, flush=True)
+    us = {}
+    for key, lst in ev.items():
+        vals = [e0.elapsed_time(e1) * 1e3 / c for (e0, e1, c) in lst]
+        us[key] = sum(vals) / len(vals)
+    return us
+
+
+def _geo(xs):
+    xs = list(xs)
+    return math.exp(sum(math.log(x) for x in xs) / len(xs))
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
+                          "(a multi-GPU run is not part of this build)"}))
+        sys.exit(2)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    specs = _specs(args.shapes)
+    modes = ["plain", "mixed"] if args.mode == "both" else [args.mode]
+    data = _inputs(specs, dev)
+    resid = {m: _gate(specs, data, m) for m in modes}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    times = {m: _time(specs, data, m, args.steps, args.warmup, dev) for m in modes}
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
-    for row in rows:
-        key = (row["batch"], row["n"])
-        us = [e0.elapsed_time(e1) * 1e3 / c for (e0, e1, c) in per_shape[key]]
-        mean = sum(us) / len(us)
+    # The headline form: mixed when it was run (BF16x3 trailing updates), else plain.
+    head = "mixed" if "mixed" in modes else modes[0]
+    rows = []
+    for spec in specs:
+        key = (spec["batch"], spec["n"])
         b, n = key
         nbytes = 8.0 * b * n * n
         flops = b * n ** 3 / 3.0
-        row.update({
-            "us": round(mean, 2),
-            "us_min": round(min(us), 2),
-            "GBps": round(nbytes / mean / 1e3, 1),
-            "TFLOPs": round(flops / mean / 1e6, 3),
-            "hbm_frac": round(nbytes / mean / 1e3 / HBM_PEAK_GBS, 4),
-            "fp32_frac": round(flops / mean / 1e6 / FP32_PEAK_TFLOPS, 4),
-        })
-    geo = math.exp(sum(math.log(r["us"]) for r in rows) / len(rows))
-    ms_per_step = sum(r["us"] for r in rows) / 1e3
+        row = {"batch": b, "n": n}
+        for m in modes:
+            us = times[m][key]
+            row[m] = {"us": round(us, 2), "GBps": round(nbytes / us / 1e3, 1),
+                      "TFLOPs": round(flops / us / 1e6, 3),
+                      "hbm_frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 4),
+                      "fp32_frac": round(flops / us / 1e6 / FP32_PEAK_TFLOPS, 4),
+                      "fp64_resid": resid[m][key]}
+        rows.append(row)
+    geo = {m: _geo(times[m][k] for k in times[m]) for m in modes}
+    ms_per_step = sum(times[head].values()) / 1e3
 
-    # Headline roofline: the single-launch wave-register kernel on 4096x32.
+    # Headline roofline: the single-launch wave-register kernel on 4096x32
+    # (one launch per call, so the per-call event time is its duration).
     roof = None
     r32 = next((r for r in rows if (r["batch"], r["n"]) == (4096, 32)), None)
     if r32 is not None:
-        roof = {"kernel": "k_potrf_small<32> (4096x32)", "bound": "hbm",
-                "achieved": r32["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(r32["GBps"] / HBM_PEAK_GBS, 4), "traffic": None}
+        g = r32[head]["GBps"]
+        roof = {"kernel": "k_potrf_small<32,vec> on 4096x32", "bound": "hbm",
+                "achieved": g, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(g / HBM_PEAK_GBS, 4), "traffic": None}
     out = {
         "metric": "geomean us over 15 batch x n SPD shapes",
-        "value": round(geo, 2),
+        "value": round(geo[head], 2),
         "unit": "us",
         "n_gpus": 1,
         "steps": args.steps,
@@ -193,20 +227,23 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if head == "plain" else "f32+bf16x3",
         "data": "synthetic",
         "config": {"workload": "15-shape batched Cholesky grid (4096x32 ... 1x32768), FP32 in/out",
-                   "mode": args.mode, "shapes": len(rows)},
+                   "form": head, "shapes": len(rows)},
         "roofline": roof,
+        "geomean_us": {m: round(geo[m], 2) for m in modes},
+        "max_fp64_resid": {m: max(resid[m].values()) for m in modes},
         "wall_s_timed": round(wall, 2),
-        "per_shape": [{k: r[k] for k in ("batch", "n", "us", "GBps", "TFLOPs", "hbm_frac", "fp32_frac",
-                                         "fp64_resid")} for r in rows],
+        "per_shape": rows,
     }
+    if len(modes) == 2:
+        out["mixed_speedup_vs_plain"] = round(geo["plain"] / geo["mixed"], 4)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget_s)
     if args.detail:
         with open(args.detail, "w") as f:
-            json.dump({"rows": rows, "total_wall_s": time.perf_counter() - t_all}, f, indent=1)
+            json.dump(out, f, indent=1)
     print(json.dumps(out))
 
 

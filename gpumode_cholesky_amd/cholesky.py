@@ -60,11 +60,11 @@ def cholesky(A: torch.Tensor, mode: str = "plain", return_info: bool = False,
     shape = A.shape
     n = int(shape[-1])
     A3 = A.reshape(-1, n, n)
-    if A3.stride(-1) != 1 or (A3.shape[0] > 1 and A3.stride(0) < n * A3.stride(1)):
-        A3 = A3.contiguous()
     batch = int(A3.shape[0])
-    ld = int(A3.stride(1)) if n > 0 else 1
-    bstride = int(A3.stride(0)) if batch > 1 else n * ld
+    # The C ABI gives A and L one layout; L is contiguous, so A must be too.
+    if not A3.is_contiguous():
+        A3 = A3.contiguous()
+    ld, bstride = max(n, 1), n * n
     if out is None:
         L = torch.empty((batch, n, n), dtype=torch.float32, device=A.device)
     else:
@@ -72,10 +72,6 @@ def cholesky(A: torch.Tensor, mode: str = "plain", return_info: bool = False,
                 or not out.is_contiguous():
             raise ValueError("out must be a contiguous FP32 tensor shaped like A")
         L = out.view(batch, n, n)
-    if ld != n:
-        # L is always contiguous; give the kernels one layout for both.
-        A3 = A3.contiguous()
-        ld, bstride = n, n * n
     info = torch.empty((batch,), dtype=torch.int32, device=A.device) if return_info else None
     ws_n = lib.chol_workspace_bytes(batch, n, _MODES[mode])
     ws = torch.empty((max(int(ws_n), 1),), dtype=torch.uint8, device=A.device)

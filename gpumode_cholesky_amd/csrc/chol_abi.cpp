@@ -4,9 +4,10 @@
 // Dispatch by order n (the reference dispatches by (batch, n) tables in
 // custom_kernel, submission.py:4101-4268; here the route depends only on n so
 // every (batch, n) takes a HIP route and there is no library fallback):
-//   n <= 64 : chol_small.hip, whole matrix in one wavefront's registers;
-//   n >  64 : chol_blocked.hip, 64-wide diagonal blocks with explicit
-//             inverses, panel GEMM and lower-tile SYRK.
+//   n <= 64        : chol_small.hip, whole matrix in one wavefront's registers;
+//   64 < n <= 256  : chol_lds.hip, whole matrix in one workgroup's LDS;
+//   n >  256       : chol_blocked.hip + chol_gemm.hip, 64-wide diagonal
+//                    blocks with explicit inverses, MFMA panel and SYRK.
 #include <cstdarg>
 #include <cstring>
 
@@ -30,12 +31,15 @@ static int outer_block(int64_t n) { return n <= 256 ? 256 : 256; }
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Largest order of the whole-matrix-in-LDS kernel.
+constexpr int64_t kLdsMaxN = 256;
+
 static size_t potrf_ws(int64_t batch, int64_t n) {
-  if (n <= 64 || batch <= 0) return 0;
+  if (n <= kLdsMaxN || batch <= 0) return 0;
   return align256((size_t)batch * kNB * kNB * sizeof(float));
 }
 
-static int blocked_potrf(const Batch& p, float* winv, hipStream_t s) {
+static int blocked_potrf(const Batch& p, float* winv, bool split, hipStream_t s) {
   int rc = launch_copy_lower(p, s);
   if (rc) return rc;
   const int n = p.n;
@@ -47,11 +51,11 @@ static int blocked_potrf(const Batch& p, float* winv, hipStream_t s) {
       if (k0 + kNB >= n) break;
       if ((rc = launch_panel(p, k0, winv, s))) return rc;
       if (k0 + kNB < K0 + W) {
-        if ((rc = launch_syrk(p, k0 + kNB, k0 + kNB, K0 + W, k0, kNB, s))) return rc;
+        if ((rc = launch_syrk(p, k0 + kNB, K0 + W, k0, kNB, split, s))) return rc;
       }
     }
     if (K0 + W < n) {
-      if ((rc = launch_syrk(p, K0 + W, K0 + W, n, K0, W, s))) return rc;
+      if ((rc = launch_syrk(p, K0 + W, n, K0, W, split, s))) return rc;
     }
   }
   return 0;
@@ -97,10 +101,6 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n, int64_t b
               (long long)n, (long long)batch_stride);
     return CHOL_E_ARG;
   }
-  if (mode == CHOL_MODE_MIXED) {
-    set_error("chol_potrf_f32: mixed mode is not available in this build");
-    return CHOL_E_MODE;
-  }
   const size_t need = potrf_ws(batch, n);
   if (need > 0 && (ws == nullptr || ws_bytes < need)) {
     set_error("chol_potrf_f32: workspace of %zu bytes required, got %zu", need, ws_bytes);
@@ -109,6 +109,7 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n, int64_t b
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   Batch p{A, L, batch, (int)n, batch > 1 ? batch_stride : n * ld, ld, info};
   if (n <= 64) return launch_small(p, s);
+  if (n <= kLdsMaxN) return launch_lds(p, s);
   int rc = launch_zero_info(info, batch, s);
   if (rc) return rc;
   // The blocked kernels put the batch on grid.y (at most 65535 blocks).
@@ -119,7 +120,7 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n, int64_t b
     q.A = A + b0 * p.bs;
     q.L = L + b0 * p.bs;
     q.info = info ? info + b0 : nullptr;
-    if ((rc = blocked_potrf(q, reinterpret_cast<float*>(ws), s))) return rc;
+    if ((rc = blocked_potrf(q, reinterpret_cast<float*>(ws), mode == CHOL_MODE_MIXED, s))) return rc;
   }
   return 0;
 }

@@ -6,6 +6,7 @@
 //           form its inverse W = L11^-1 (kept in the workspace);
 //   panel : L21 = A21 W^T for every row below the block (a GEMM, K = 64);
 //   syrk  : C -= P P^T on the lower trailing tiles.
+// The two products run on the matrix cores (chol_gemm.hip).
 // Inner 64-blocks are grouped into an outer block of `nbo` columns: inner
 // steps update only the outer block's columns, and one SYRK with K = nbo then
 // updates the rest of the trailing matrix, so the dominant update streams
@@ -152,131 +153,6 @@ __global__ __launch_bounds__(kWave) void k_diag_factor(Batch p, int k0, float* _
   for (int i = 0; i < NB; ++i) Wb[i * NB + c] = w[i];
 }
 
-// --------------------------------------------------------------- panel ----
-// L[i, k0:k0+64] = A21[i, :] W^T for rows i in [k0+64, n), 64 rows per block.
-// W is lower-triangular, so out[i][j] = sum_{c <= j} A21[i][c] W[j][c].
-__global__ __launch_bounds__(256) void k_panel(Batch p, int k0, const float* __restrict__ winv) {
-  constexpr int NB = kNB;
-  constexpr int P = 64 + 4;
-  __shared__ __align__(16) float As[NB * P];  // As[c][row]
-  __shared__ __align__(16) float Ws[NB * P];  // Ws[c][col] = W[col][c]
-  const int tid = threadIdx.x;
-  const int64_t b = blockIdx.y;
-  const int row0 = k0 + NB + blockIdx.x * 64;
-  const int n = p.n;
-  float* Lb = p.L + b * p.bs;
-  const float* Wb = winv + b * (NB * NB);
-
-  // Load: thread t covers row t/4 and 16 columns (t%4)*16 .. +15.
-  {
-    const int rr = tid >> 2;
-    const int c0 = (tid & 3) * 16;
-    const int gi = row0 + rr;
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int cc = c0 + u;
-      As[cc * P + rr] = (gi < n) ? Lb[(int64_t)gi * p.ld + k0 + cc] : 0.f;
-      Ws[cc * P + rr] = Wb[rr * NB + cc];
-    }
-  }
-  __syncthreads();
-  const int tx = tid & 15;  // output cols tx*4 .. +3
-  const int ty = tid >> 4;  // output rows ty*4 .. +3
-  float acc[4][4] = {};
-#pragma unroll 8
-  for (int c = 0; c < NB; ++c) {
-    const float4 av = *reinterpret_cast<const float4*>(&As[c * P + ty * 4]);
-    const float4 wv = *reinterpret_cast<const float4*>(&Ws[c * P + tx * 4]);
-    const float ar[4] = {av.x, av.y, av.z, av.w};
-    const float wr[4] = {wv.x, wv.y, wv.z, wv.w};
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(ar[i], wr[j], acc[i][j]);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int gi = row0 + ty * 4 + i;
-    if (gi >= n) continue;
-    float* dst = Lb + (int64_t)gi * p.ld + k0 + tx * 4;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) dst[j] = acc[i][j];
-  }
-}
-
-// ---------------------------------------------------------------- syrk ----
-// C[i][j] -= sum_{c in [k0, k0+kl)} L[i][c] L[j][c]
-// for j in [col0, col1), i in [j, n).  64x64 tiles (ti >= tj) over the
-// lower region; tile (ti, tj) covers rows col0+64*ti.., cols col0+64*tj...
-// Grid.x enumerates tiles column-major: column tj holds TI - tj tiles.
-__device__ __forceinline__ void decode_tile(int idx, int TI, int& ti, int& tj) {
-  // Tiles before column t: S(t) = t*TI - t*(t-1)/2.  Solve S(t) <= idx.
-  const float twoTI1 = 2.f * TI + 1.f;
-  const float disc = fmaxf(twoTI1 * twoTI1 - 8.f * (float)idx, 0.f);
-  int t = (int)floorf((twoTI1 - sqrtf(disc)) * 0.5f);
-  if (t < 0) t = 0;
-  while (t > 0 && (int64_t)t * TI - (int64_t)t * (t - 1) / 2 > idx) --t;
-  while ((int64_t)(t + 1) * TI - (int64_t)(t + 1) * t / 2 <= idx) ++t;
-  tj = t;
-  ti = t + (idx - (int)((int64_t)t * TI - (int64_t)t * (t - 1) / 2));
-}
-
-__global__ __launch_bounds__(256) void k_syrk(Batch p, int col0, int col1, int k0, int kl, int TI) {
-  constexpr int KC = 16;
-  constexpr int P = 64 + 4;
-  __shared__ __align__(16) float As[KC * P];  // As[c][i]
-  __shared__ __align__(16) float Bs[KC * P];  // Bs[c][j]
-  const int tid = threadIdx.x;
-  const int64_t b = blockIdx.y;
-  int ti, tj;
-  decode_tile(blockIdx.x, TI, ti, tj);
-  const int i0 = col0 + 64 * ti;
-  const int j0 = col0 + 64 * tj;
-  const int n = p.n;
-  float* Lb = p.L + b * p.bs;
-
-  const int tx = tid & 15;
-  const int ty = tid >> 4;
-  float acc[4][4] = {};
-  const int lr = tid >> 2;        // load row 0..63
-  const int lc = (tid & 3) * 4;   // load cols lc..lc+3 within the chunk
-  const int gi = i0 + lr;
-  const int gj = j0 + lr;
-  for (int kc = 0; kc < kl; kc += KC) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int c = kc + lc + u;
-      const bool kin = c < kl;
-      As[(lc + u) * P + lr] = (gi < n && kin) ? Lb[(int64_t)gi * p.ld + k0 + c] : 0.f;
-      Bs[(lc + u) * P + lr] = (gj < col1 && kin) ? Lb[(int64_t)gj * p.ld + k0 + c] : 0.f;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      const float4 av = *reinterpret_cast<const float4*>(&As[c * P + ty * 4]);
-      const float4 bv = *reinterpret_cast<const float4*>(&Bs[c * P + tx * 4]);
-      const float ar[4] = {av.x, av.y, av.z, av.w};
-      const float br[4] = {bv.x, bv.y, bv.z, bv.w};
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(ar[i], br[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int ri = i0 + ty * 4 + i;
-    if (ri >= n) continue;
-    float* dst = Lb + (int64_t)ri * p.ld;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int cj = j0 + tx * 4 + j;
-      if (cj < col1 && cj <= ri) dst[cj] -= acc[i][j];
-    }
-  }
-}
-
 }  // namespace
 
 int launch_zero_info(int32_t* info, int64_t batch, hipStream_t s) {
@@ -301,26 +177,6 @@ int launch_copy_lower(const Batch& p, hipStream_t s) {
 int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s) {
   hipLaunchKernelGGL(k_diag_factor, dim3((unsigned)p.batch), dim3(kWave), 0, s, p, k0, winv);
   return launch_status("k_diag_factor");
-}
-
-int launch_panel(const Batch& p, int k0, const float* winv, hipStream_t s) {
-  const int rows = p.n - (k0 + kNB);
-  if (rows <= 0) return 0;
-  hipLaunchKernelGGL(k_panel, dim3((unsigned)((rows + 63) / 64), (unsigned)p.batch), dim3(256), 0, s, p,
-                     k0, winv);
-  return launch_status("k_panel");
-}
-
-int launch_syrk(const Batch& p, int row0, int col0, int col1, int k0, int kl, hipStream_t s) {
-  (void)row0;  // the update region starts at its first column (row0 == col0)
-  if (col1 <= col0 || kl <= 0) return 0;
-  const int TI = (p.n - col0 + 63) / 64;
-  const int TJ = (col1 - col0 + 63) / 64;
-  // Tiles: sum over tj < TJ of (TI - tj).
-  const int64_t tiles = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
-  hipLaunchKernelGGL(k_syrk, dim3((unsigned)tiles, (unsigned)p.batch), dim3(256), 0, s, p, col0, col1,
-                     k0, kl, TI);
-  return launch_status("k_syrk");
 }
 
 }  // namespace chol

@@ -50,13 +50,16 @@ inline int launch_status(const char* what) {
 
 // Launchers implemented in the kernel translation units.
 int launch_small(const Batch& p, hipStream_t s);                  // n <= 64
+int launch_lds(const Batch& p, hipStream_t s);                    // n <= 256
 int launch_copy_lower(const Batch& p, hipStream_t s);             // L = tril(A)
 int launch_diag_factor(const Batch& p, int k0, float* winv,
                        hipStream_t s);                            // 64x64 block
 int launch_panel(const Batch& p, int k0, const float* winv,
                  hipStream_t s);                                  // L21 = A21 W^T
-int launch_syrk(const Batch& p, int row0, int col0, int col1, int k0, int kl,
-                hipStream_t s);                                   // C -= P P^T
+// C -= P P^T over {col0 <= j < col1, j <= i < n}, K = [k0, k0+kl);
+// split = BF16x3 products (mixed mode) instead of FP32.
+int launch_syrk(const Batch& p, int col0, int col1, int k0, int kl, bool split,
+                hipStream_t s);
 int launch_zero_info(int32_t* info, int64_t batch, hipStream_t s);
 
 // Diagonal block width of the blocked path.

@@ -1,0 +1,363 @@
+// chol_gemm.hip — FP32 MFMA products of the blocked path (gfx950).
+//
+//   syrk  : C[i][j] -= sum_k P[i][k] P[j][k]   over the lower trailing tiles
+//   panel : L21 = A21 W^T                        (in place, K = 64)
+//
+// Both are C (op)= A B^T with A, B row-major along K, run on
+// v_mfma_f32_16x16x4_f32.  A workgroup of 4 wavefronts computes a BM x BN
+// tile; each wavefront a (BM/2) x (BN/2) quarter as 16x16 MFMA blocks.  K is
+// staged through LDS in chunks of 32 (rows padded to 36 floats), double
+// buffered: the global loads of chunk c+1 are in flight while chunk c is
+// multiplied.  Within a 16-wide K slice lane group g = lane/16 owns
+// k = 4g..4g+3, so one ds_read_b128 feeds four MFMA steps (the sum over k is
+// the same set of products in a different order).
+//
+// MFMA 16x16x4 f32 operand layout (checked on gfx950): A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16*(i/4), element i%4.
+//
+// Reference counterparts: the trailing baddbmm_ / addmm_ SYRK-as-GEMM and
+// the panel-times-inverse bmm of the reference's blocked drivers
+// (submission.py:5163-5184, :2529-2534, :3362-3401), which ran on TF32/FP16
+// tensor cores; here all-FP32 matrix cores.
+#include "chol_common.h"
+
+namespace chol {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int KC = 32;        // K chunk staged in LDS
+constexpr int KP = KC + 4;    // LDS row pitch (floats)
+constexpr int kThreads = 256;
+
+template <int ROWS, bool VEC>
+__device__ __forceinline__ void load_rows(float4 (&reg)[ROWS * KC / 4 / kThreads], const float* base,
+                                          int64_t ld, int row0, int rlim, int k, int tid) {
+  // ROWS x KC floats as float4 chunks; thread t takes chunks t, t+256, ...
+  constexpr int PER = ROWS * KC / 4 / kThreads;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int c = tid + u * kThreads;
+    const int r = c / (KC / 4);
+    const int q = (c % (KC / 4)) * 4;
+    const int gr = row0 + r;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (gr < rlim) {
+      const float* src = base + (int64_t)gr * ld + k + q;
+      if constexpr (VEC) {
+        v = *reinterpret_cast<const float4*>(src);
+      } else {
+        v.x = src[0];
+        v.y = src[1];
+        v.z = src[2];
+        v.w = src[3];
+      }
+    }
+    reg[u] = v;
+  }
+}
+
+template <int ROWS>
+__device__ __forceinline__ void store_rows(float* __restrict__ lds, const float4 (&reg)[ROWS * KC / 4 / kThreads],
+                                           int tid) {
+  constexpr int PER = ROWS * KC / 4 / kThreads;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int c = tid + u * kThreads;
+    const int r = c / (KC / 4);
+    const int q = (c % (KC / 4)) * 4;
+    *reinterpret_cast<float4*>(lds + r * KP + q) = reg[u];
+  }
+}
+
+// MODE 0: C -= A B^T on the lower part (i >= j), j < jlim, i < ilim.
+// MODE 1: C  = A B^T, i < ilim (panel; A and C alias row-for-row).
+template <int BM, int BN, int MODE, bool VEC>
+__device__ __forceinline__ void gemm_tile(const float* Abase, int64_t lda, int a_row0, int a_lim,
+                                          int ka, const float* Bbase, int64_t ldb, int b_row0,
+                                          int b_lim, int kb, int K, float* Cbase, int64_t ldc,
+                                          int c_row0, int c_col0, int c_ilim, int c_jlim) {
+  constexpr int WM = BM / 2, WN = BN / 2;      // per-wave tile
+  constexpr int FM = WM / 16, FN = WN / 16;    // MFMA blocks per wave
+  constexpr int PA = BM * KC / 4 / kThreads;
+  constexpr int PB = BN * KC / 4 / kThreads;
+  __shared__ __align__(16) float As[2][BM * KP];
+  __shared__ __align__(16) float Bs[2][BN * KP];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * WM;
+  const int wn = (wave & 1) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+
+  floatx4 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  float4 ra[PA], rb[PB];
+  load_rows<BM, VEC>(ra, Abase, lda, a_row0, a_lim, ka, tid);
+  load_rows<BN, VEC>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
+  store_rows<BM>(As[0], ra, tid);
+  store_rows<BN>(Bs[0], rb, tid);
+  __syncthreads();
+
+  const int nchunks = K / KC;
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int cur = ch & 1;
+    const bool more = ch + 1 < nchunks;
+    if (more) {
+      load_rows<BM, VEC>(ra, Abase, lda, a_row0, a_lim, ka + (ch + 1) * KC, tid);
+      load_rows<BN, VEC>(rb, Bbase, ldb, b_row0, b_lim, kb + (ch + 1) * KC, tid);
+    }
+    const float* as = As[cur];
+    const float* bs = Bs[cur];
+#pragma unroll
+    for (int ks = 0; ks < KC; ks += 16) {
+      float4 fa[FM], fb[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+        fa[a] = *reinterpret_cast<const float4*>(as + (wm + a * 16 + li) * KP + ks + 4 * lg);
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+        fb[b] = *reinterpret_cast<const float4*>(bs + (wn + b * 16 + li) * KP + ks + 4 * lg);
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].x, fb[b].x, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].y, fb[b].y, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].z, fb[b].z, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].w, fb[b].w, acc[a][b], 0, 0, 0);
+        }
+    }
+    if (more) {
+      store_rows<BM>(As[cur ^ 1], ra, tid);
+      store_rows<BN>(Bs[cur ^ 1], rb, tid);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: lane holds D[4*lg + v][li] of each 16x16 block.
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) {
+      const int j = c_col0 + wn + b * 16 + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = c_row0 + wm + a * 16 + 4 * lg + v;
+        if (i >= c_ilim) continue;
+        float* dst = Cbase + (int64_t)i * ldc + j;
+        if constexpr (MODE == 0) {
+          if (j < c_jlim && j <= i) *dst -= acc[a][b][v];
+        } else {
+          *dst = acc[a][b][v];
+        }
+      }
+    }
+}
+
+// ------------------------------------------------------------ BF16x3 ----
+// Mixed form of the SYRK: each FP32 operand x is split when staged into LDS
+// as x = hi + lo (hi = bf16(x), lo = bf16(x - hi), v_cvt_pk_bf16_f32
+// rounding to nearest even), and A B^T is accumulated in FP32 from three
+// v_mfma_f32_16x16x32_bf16 products hiA hiB^T + hiA loB^T + loA hiB^T.
+// The dropped loA loB^T term and the split remainder are ~2^-16 of |a b|.
+// Operand layout of 16x16x32 bf16: A[i][k] for k = 8g..8g+7 in lane i + 16g,
+// B likewise by column; D as for 16x16x4 f32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+constexpr int KPH = KC + 8;  // bf16 LDS row pitch (80 bytes)
+
+template <int ROWS>
+__device__ __forceinline__ void store_rows_split(__bf16* __restrict__ hi, __bf16* __restrict__ lo,
+                                                 const float4 (&reg)[ROWS * KC / 4 / kThreads], int tid) {
+  constexpr int PER = ROWS * KC / 4 / kThreads;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int c = tid + u * kThreads;
+    const int r = c / (KC / 4);
+    const int q = (c % (KC / 4)) * 4;
+    const float4 v = reg[u];
+    const __bf16 h0 = (__bf16)v.x, h1 = (__bf16)v.y, h2 = (__bf16)v.z, h3 = (__bf16)v.w;
+    const __bf16 l0 = (__bf16)(v.x - (float)h0), l1 = (__bf16)(v.y - (float)h1);
+    const __bf16 l2 = (__bf16)(v.z - (float)h2), l3 = (__bf16)(v.w - (float)h3);
+    bf16x2* ph = reinterpret_cast<bf16x2*>(hi + r * KPH + q);
+    bf16x2* pl = reinterpret_cast<bf16x2*>(lo + r * KPH + q);
+    ph[0] = bf16x2{h0, h1};
+    ph[1] = bf16x2{h2, h3};
+    pl[0] = bf16x2{l0, l1};
+    pl[1] = bf16x2{l2, l3};
+  }
+}
+
+// C -= A B^T on the lower part (i >= j), j < jlim, i < ilim, BF16x3 products.
+template <int BM, int BN, bool VEC>
+__device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda, int a_row0, int a_lim, int ka,
+                                                 const float* Bbase, int64_t ldb, int b_row0, int b_lim, int kb,
+                                                 int K, float* Cbase, int64_t ldc, int c_row0, int c_col0,
+                                                 int c_ilim, int c_jlim) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int PA = BM * KC / 4 / kThreads;
+  constexpr int PB = BN * KC / 4 / kThreads;
+  __shared__ __align__(16) __bf16 Ah[2][BM * KPH];
+  __shared__ __align__(16) __bf16 Al[2][BM * KPH];
+  __shared__ __align__(16) __bf16 Bh[2][BN * KPH];
+  __shared__ __align__(16) __bf16 Bl[2][BN * KPH];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * WM;
+  const int wn = (wave & 1) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+
+  floatx4 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  float4 ra[PA], rb[PB];
+  load_rows<BM, VEC>(ra, Abase, lda, a_row0, a_lim, ka, tid);
+  load_rows<BN, VEC>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
+  store_rows_split<BM>(Ah[0], Al[0], ra, tid);
+  store_rows_split<BN>(Bh[0], Bl[0], rb, tid);
+  __syncthreads();
+
+  const int nchunks = K / KC;
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int cur = ch & 1;
+    const bool more = ch + 1 < nchunks;
+    if (more) {
+      load_rows<BM, VEC>(ra, Abase, lda, a_row0, a_lim, ka + (ch + 1) * KC, tid);
+      load_rows<BN, VEC>(rb, Bbase, ldb, b_row0, b_lim, kb + (ch + 1) * KC, tid);
+    }
+    bf16x8 fah[FM], fal[FM], fbh[FN], fbl[FN];
+#pragma unroll
+    for (int a = 0; a < FM; ++a) {
+      const int off = (wm + a * 16 + li) * KPH + 8 * lg;
+      fah[a] = *reinterpret_cast<const bf16x8*>(&Ah[cur][off]);
+      fal[a] = *reinterpret_cast<const bf16x8*>(&Al[cur][off]);
+    }
+#pragma unroll
+    for (int b = 0; b < FN; ++b) {
+      const int off = (wn + b * 16 + li) * KPH + 8 * lg;
+      fbh[b] = *reinterpret_cast<const bf16x8*>(&Bh[cur][off]);
+      fbl[b] = *reinterpret_cast<const bf16x8*>(&Bl[cur][off]);
+    }
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fal[a], fbh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbh[b], acc[a][b], 0, 0, 0);
+      }
+    if (more) {
+      store_rows_split<BM>(Ah[cur ^ 1], Al[cur ^ 1], ra, tid);
+      store_rows_split<BN>(Bh[cur ^ 1], Bl[cur ^ 1], rb, tid);
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) {
+      const int j = c_col0 + wn + b * 16 + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = c_row0 + wm + a * 16 + 4 * lg + v;
+        if (i < c_ilim && j < c_jlim && j <= i) Cbase[(int64_t)i * ldc + j] -= acc[a][b][v];
+      }
+    }
+}
+
+// Tiles of the lower region {(i, j): col0 <= j < col1, j <= i < n} in 128x128
+// tiles (ti >= tj), enumerated column by column: column tj holds TI - tj.
+__device__ __forceinline__ void decode_tile(int idx, int TI, int& ti, int& tj) {
+  const float twoTI1 = 2.f * TI + 1.f;
+  const float disc = fmaxf(twoTI1 * twoTI1 - 8.f * (float)idx, 0.f);
+  int t = (int)floorf((twoTI1 - sqrtf(disc)) * 0.5f);
+  if (t < 0) t = 0;
+  while (t > 0 && (int64_t)t * TI - (int64_t)t * (t - 1) / 2 > idx) --t;
+  while ((int64_t)(t + 1) * TI - (int64_t)(t + 1) * t / 2 <= idx) ++t;
+  tj = t;
+  ti = t + (idx - (int)((int64_t)t * TI - (int64_t)t * (t - 1) / 2));
+}
+
+constexpr int ST = 128;  // SYRK tile
+
+template <bool VEC, bool SPLIT>
+__global__ __launch_bounds__(kThreads) void k_syrk_mfma(Batch p, int col0, int col1, int k0, int kl, int TI) {
+  int ti, tj;
+  decode_tile(blockIdx.x, TI, ti, tj);
+  const int64_t b = blockIdx.y;
+  float* Lb = p.L + b * p.bs;
+  const int i0 = col0 + ST * ti;
+  const int j0 = col0 + ST * tj;
+  if constexpr (SPLIT)
+    gemm_tile_bf16x3<ST, ST, VEC>(Lb, p.ld, i0, p.n, k0, Lb, p.ld, j0, col1, k0, kl, Lb, p.ld, i0, j0, p.n, col1);
+  else
+    gemm_tile<ST, ST, 0, VEC>(Lb, p.ld, i0, p.n, k0, Lb, p.ld, j0, col1, k0, kl, Lb, p.ld, i0, j0, p.n, col1);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(kThreads) void k_panel_mfma(Batch p, int k0, const float* __restrict__ winv) {
+  const int64_t b = blockIdx.y;
+  float* Lb = p.L + b * p.bs;
+  const int i0 = k0 + kNB + 128 * blockIdx.x;
+  const float* Wb = winv + b * (kNB * kNB);
+  gemm_tile<128, kNB, 1, VEC>(Lb, p.ld, i0, p.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, p.ld, i0, k0, p.n, k0 + kNB);
+}
+
+bool vec_ok(const Batch& p) {
+  return (reinterpret_cast<uintptr_t>(p.L) & 15) == 0 && (p.ld % 4) == 0 && (p.bs % 4) == 0;
+}
+
+}  // namespace
+
+int launch_panel(const Batch& p, int k0, const float* winv, hipStream_t s) {
+  const int rows = p.n - (k0 + kNB);
+  if (rows <= 0) return 0;
+  dim3 grid((unsigned)((rows + 127) / 128), (unsigned)p.batch);
+  if (vec_ok(p))
+    hipLaunchKernelGGL(k_panel_mfma<true>, grid, dim3(kThreads), 0, s, p, k0, winv);
+  else
+    hipLaunchKernelGGL(k_panel_mfma<false>, grid, dim3(kThreads), 0, s, p, k0, winv);
+  return launch_status("k_panel_mfma");
+}
+
+int launch_syrk(const Batch& p, int col0, int col1, int k0, int kl, bool split, hipStream_t s) {
+  if (col1 <= col0 || kl <= 0) return 0;
+  if (kl % KC != 0) {
+    set_error("launch_syrk: K=%d is not a multiple of %d", kl, KC);
+    return CHOL_E_ARG;
+  }
+  const int TI = (p.n - col0 + ST - 1) / ST;
+  const int TJ = (col1 - col0 + ST - 1) / ST;
+  const int64_t tiles = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
+  dim3 grid((unsigned)tiles, (unsigned)p.batch);
+  const bool vec = vec_ok(p);
+  if (split) {
+    if (vec)
+      hipLaunchKernelGGL((k_syrk_mfma<true, true>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+    else
+      hipLaunchKernelGGL((k_syrk_mfma<false, true>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL((k_syrk_mfma<true, false>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+    else
+      hipLaunchKernelGGL((k_syrk_mfma<false, false>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+  }
+  return launch_status("k_syrk_mfma");
+}
+
+}  // namespace chol

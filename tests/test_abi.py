@@ -65,13 +65,15 @@ def test_rejects_bad_mode_and_strides():
 
 def test_rejects_short_workspace():
     lib = _lib.load()
-    need = lib.chol_workspace_bytes(4, 200, 0)
-    assert need >= 4 * 64 * 64 * 4
-    assert _potrf(batch=4, n=200, ld=200, bs=200 * 200) == _lib.E_WORKSPACE
-    assert _potrf(batch=4, n=200, ld=200, bs=200 * 200, ws=ctypes.c_void_p(0x3000),
+    need = lib.chol_workspace_bytes(4, 300, 0)
+    assert need >= 4 * 64 * 64 * 4  # one 64x64 diagonal-block inverse per matrix
+    assert _potrf(batch=4, n=300, ld=300, bs=300 * 300) == _lib.E_WORKSPACE
+    assert _potrf(batch=4, n=300, ld=300, bs=300 * 300, ws=ctypes.c_void_p(0x3000),
                   wsb=need - 1) == _lib.E_WORKSPACE
     assert "workspace" in _lib.last_error()
-    assert lib.chol_workspace_bytes(4096, 32, 0) == 0  # wave-register path needs none
+    # whole-matrix paths (registers, LDS) need none
+    assert lib.chol_workspace_bytes(4096, 32, 0) == 0
+    assert lib.chol_workspace_bytes(64, 256, 0) == 0
 
 
 def test_custom_kernel_refuses_cpu_tensors():

@@ -4,9 +4,9 @@ and against torch.linalg.cholesky on the same device.
 Tolerances (FP32 arithmetic, stated here and used below):
   * reconstruction: ||L L^T - A||_F / ||A||_F (FP64) <= RESID_C * sqrt(n) * eps32
   * forward error:  ||L - L64||_max / ||L64||_max <= FWD_C * e_torch + FWD_ABS,
-    where e_torch is the same error of torch.linalg.cholesky on the same GPU
-    (the reference's own ref_kernel, reference/reference.py:129-130) and L64
-    the FP64 oracle factor;
+    where e_torch is the same error of torch.linalg.cholesky in FP32 (the
+    reference's own ref_kernel, reference/reference.py:129-130, run on the
+    host) and L64 the FP64 oracle factor;
   * the reference's gate (reference/reference.py:137-188) must pass;
   * exact zeros above the diagonal, positive diagonal, input unchanged.
 """
@@ -26,7 +26,7 @@ EPS32 = float(np.finfo(np.float32).eps)
 RESID_C = 8.0
 FWD_C = 20.0
 FWD_ABS = 8 * EPS32
-DEV = torch.device("cuda")
+DEV = torch.device("cuda", 0)
 
 
 def _say(msg):
@@ -48,7 +48,11 @@ def _check_against_oracle(oracle_mod, a_cpu, label):
     L64, info = oracle_mod.potrf64(a_cpu.numpy())
     assert (info == 0).all()
     fro, _, _ = oracle_mod.residual(a_cpu.numpy(), Lc)
-    Lt = torch.linalg.cholesky(a).cpu().numpy().astype(np.float64)
+    # torch.linalg.cholesky in FP32 on the host (LAPACK spotrf): the
+    # reference's own path on a CPU box (submission.py:4268).  The GPU
+    # rocSOLVER call is kept to the benchmark shapes (test_bench_shapes_gate):
+    # at n = 300 and 320 it failed on this stack with hipErrorLaunchFailure.
+    Lt = torch.linalg.cholesky(a_cpu).numpy().astype(np.float64)
     scale = np.abs(L64).reshape(L64.shape[0], -1).max(axis=1)
     e_ours = (np.abs(Lc - L64).reshape(L64.shape[0], -1).max(axis=1) / scale).max()
     e_torch = (np.abs(Lt - L64).reshape(L64.shape[0], -1).max(axis=1) / scale).max()
@@ -117,6 +121,46 @@ def test_bench_shapes_gate(spec):
     _say(line)
     assert resid <= RESID_C * math.sqrt(n) * EPS32
     assert torch.count_nonzero(torch.triu(L, 1)).item() == 0
+    del a, L
+    torch.cuda.empty_cache()
+
+
+MIXED_RESID = 1.0e-5  # ||LL^T - A||_F / ||A||_F target of the mixed form
+
+
+@pytest.mark.parametrize("batch,n,case", [(3, 300, "dense"), (2, 512, "spectrum"), (2, 1024, "dense"),
+                                          (2, 1024, "spectrum"), (1, 2048, "rowscale")])
+def test_mixed_against_oracle(oracle_mod, batch, n, case):
+    # BF16x3 trailing updates: residual within MIXED_RESID of the FP64
+    # oracle's factorisation target, the reference gate passes, and the
+    # factor stays within 1e-3 (relative, max-norm) of the FP64 oracle factor.
+    cond = {"dense": 2, "spectrum": 5, "rowscale": 4}[case]
+    a = harness.generate_input(batch, n, cond, 3000 + n, case, device="cpu")
+    L = cholesky(a.to(DEV), mode="mixed")
+    ok, msg = harness.check_implementation(a.to(DEV), L)
+    assert ok, msg
+    Lc = L.cpu().numpy()
+    fro, _, _ = oracle_mod.residual(a.numpy(), Lc)
+    L64, _ = oracle_mod.potrf64(a.numpy())
+    e = np.abs(Lc - L64).max() / np.abs(L64).max()
+    _say(f"[mixed] {batch}x{n} {case}: resid={fro.max():.3e} fwd={e:.3e} {msg}")
+    assert fro.max() <= MIXED_RESID
+    assert e <= 1e-3
+    assert (np.triu(Lc, 1) == 0).all()
+
+
+@pytest.mark.parametrize("spec", [s for s in harness.BENCH_SPECS if s["n"] > 256],
+                         ids=lambda s: f"{s['batch']}x{s['n']}")
+def test_mixed_bench_shapes(spec):
+    b, n = spec["batch"], spec["n"]
+    _say(f"[mixed-shape] start {b}x{n}")
+    a = harness.config_input(b, n, spec["seed"], device=DEV)
+    L = cholesky(a, mode="mixed")
+    ok, msg = harness.check_implementation(a, L)
+    assert ok, msg
+    resid = harness.fp64_residual(a, L)
+    _say(f"[mixed-shape] {b}x{n}: {msg} fp64_resid={resid:.3e}")
+    assert resid <= MIXED_RESID
     del a, L
     torch.cuda.empty_cache()
 
