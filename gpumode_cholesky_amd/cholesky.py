@@ -14,7 +14,7 @@ minor.
 """
 from __future__ import annotations
 
-import ctypes
+import math
 
 import torch
 
@@ -22,9 +22,11 @@ from . import _lib
 
 _MODES = {"plain": _lib.MODE_PLAIN, "mixed": _lib.MODE_MIXED}
 
-
-def _stream_ptr(device: torch.device) -> int:
-    return torch.cuda.current_stream(device).cuda_stream
+try:  # raw hipStream_t of the current stream, without building a Stream object
+    _raw_stream = torch._C._cuda_getCurrentRawStream
+except AttributeError:  # pragma: no cover - older torch
+    def _raw_stream(index):
+        return torch.cuda.current_stream(index).cuda_stream
 
 
 def _validate(A: torch.Tensor) -> torch.Tensor:
@@ -49,43 +51,44 @@ def cholesky(A: torch.Tensor, mode: str = "plain", return_info: bool = False,
              out: torch.Tensor | None = None):
     """Lower Cholesky factor of each matrix of ``A`` ([..., n, n] FP32).
 
-    mode: ``"plain"`` (all-FP32) or ``"mixed"`` (split-BF16 trailing updates
-    plus a correction step; n >= 1024 only differs).
+    mode: ``"plain"`` (all-FP32) or ``"mixed"`` (BF16x3 split products in the
+    trailing updates; differs from plain for n > 256 only).
     Returns ``L`` or ``(L, info)`` with ``info`` int32 [...] on the device.
     """
-    if mode not in _MODES:
+    m = _MODES.get(mode)
+    if m is None:
         raise ValueError(f"mode must be one of {sorted(_MODES)}, got {mode!r}")
-    A = _validate(A)
+    _validate(A)
     lib = _lib.load()
     shape = A.shape
-    n = int(shape[-1])
-    A3 = A.reshape(-1, n, n)
-    batch = int(A3.shape[0])
+    n = shape[-1]
     # The C ABI gives A and L one layout; L is contiguous, so A must be too.
-    if not A3.is_contiguous():
-        A3 = A3.contiguous()
-    ld, bstride = max(n, 1), n * n
+    A3 = A if A.is_contiguous() else A.contiguous()
+    batch = math.prod(shape[:-2])
     if out is None:
-        L = torch.empty((batch, n, n), dtype=torch.float32, device=A.device)
+        L = torch.empty(shape, dtype=torch.float32, device=A.device)
     else:
         if out.shape != shape or out.dtype != torch.float32 or out.device != A.device \
                 or not out.is_contiguous():
             raise ValueError("out must be a contiguous FP32 tensor shaped like A")
-        L = out.view(batch, n, n)
-    info = torch.empty((batch,), dtype=torch.int32, device=A.device) if return_info else None
-    ws_n = lib.chol_workspace_bytes(batch, n, _MODES[mode])
-    ws = torch.empty((max(int(ws_n), 1),), dtype=torch.uint8, device=A.device)
-    with torch.cuda.device(A.device):
-        rc = lib.chol_potrf_f32(
-            ctypes.c_void_p(A3.data_ptr()), ctypes.c_void_p(L.data_ptr()),
-            batch, n, bstride, ld, _MODES[mode],
-            ctypes.c_void_p(info.data_ptr() if info is not None else 0),
-            ctypes.c_void_p(ws.data_ptr()), int(ws_n),
-            ctypes.c_void_p(_stream_ptr(A.device)))
-    _lib.check(rc, "chol_potrf_f32")
-    L = L.view(shape)
+        L = out
+    info = torch.empty(shape[:-2], dtype=torch.int32, device=A.device) if return_info else None
+    ws_n = lib.chol_workspace_bytes(batch, n, m)
+    ws = torch.empty((ws_n,), dtype=torch.uint8, device=A.device) if ws_n else None
+    dev = A.device.index
+    if dev != torch.cuda.current_device():
+        with torch.cuda.device(dev):
+            rc = lib.chol_potrf_f32(A3.data_ptr(), L.data_ptr(), batch, n, n * n, max(n, 1), m,
+                                    info.data_ptr() if info is not None else None,
+                                    ws.data_ptr() if ws is not None else None, ws_n, _raw_stream(dev))
+    else:
+        rc = lib.chol_potrf_f32(A3.data_ptr(), L.data_ptr(), batch, n, n * n, max(n, 1), m,
+                                info.data_ptr() if info is not None else None,
+                                ws.data_ptr() if ws is not None else None, ws_n, _raw_stream(dev))
+    if rc:
+        _lib.check(rc, "chol_potrf_f32")
     if return_info:
-        return L, info.view(shape[:-2])
+        return L, info
     return L
 
 

@@ -139,30 +139,59 @@ int chol_potrf_ragged_f32(const float* const* A, float* const* L, const int64_t*
 }
 
 size_t chol_backward_workspace_bytes(int64_t batch, int64_t n) {
-  (void)batch;
-  (void)n;
+  if (batch <= 0 || n <= 0) return 0;
+  // P, T and (when L^-1 is not supplied) the recomputed inverse.
+  return align256((size_t)3 * batch * n * n * sizeof(float));
+}
+
+static int check_square_batch(const char* who, const void* a, const void* b, int64_t batch, int64_t n) {
+  if (batch < 0 || n < 0 || n > 32768) {
+    set_error("%s: batch=%lld n=%lld out of range", who, (long long)batch, (long long)n);
+    return CHOL_E_ARG;
+  }
+  if (batch > 0 && n > 0 && (!a || !b)) {
+    set_error("%s: null pointer", who);
+    return CHOL_E_ARG;
+  }
   return 0;
 }
 
 int chol_backward_f32(const float* Lm, const float* Lbar, float* Abar, int64_t batch, int64_t n, const float* Linv,
                       int32_t have_linv, void* ws, size_t ws_bytes, void* stream) {
-  (void)Lm; (void)Lbar; (void)Abar; (void)batch; (void)n; (void)Linv; (void)have_linv; (void)ws; (void)ws_bytes;
-  (void)stream;
-  set_error("chol_backward_f32: not available in this build");
-  return CHOL_E_ARG;
+  clear_error();
+  int rc = check_square_batch("chol_backward_f32", Lm, Abar, batch, n);
+  if (rc) return rc;
+  if (batch == 0 || n == 0) return 0;
+  if (!Lbar || (have_linv && !Linv)) {
+    set_error("chol_backward_f32: null Lbar, or have_linv set without Linv");
+    return CHOL_E_ARG;
+  }
+  const size_t need = (size_t)(have_linv ? 2 : 3) * batch * n * n * sizeof(float);
+  if (!ws || ws_bytes < need) {
+    set_error("chol_backward_f32: workspace of %zu bytes required, got %zu", need, ws_bytes);
+    return CHOL_E_WORKSPACE;
+  }
+  return backward(Lm, Lbar, Abar, batch, (int)n, have_linv ? Linv : nullptr, reinterpret_cast<float*>(ws),
+                  reinterpret_cast<hipStream_t>(stream));
 }
 
 size_t chol_tri_inverse_workspace_bytes(int64_t batch, int64_t n) {
-  (void)batch;
-  (void)n;
-  return 0;
+  if (batch <= 0 || n <= 0) return 0;
+  return align256((size_t)batch * n * n * sizeof(float));
 }
 
 int chol_tri_inverse_f32(const float* Lm, float* Linv, int64_t batch, int64_t n, void* ws, size_t ws_bytes,
                          void* stream) {
-  (void)Lm; (void)Linv; (void)batch; (void)n; (void)ws; (void)ws_bytes; (void)stream;
-  set_error("chol_tri_inverse_f32: not available in this build");
-  return CHOL_E_ARG;
+  clear_error();
+  int rc = check_square_batch("chol_tri_inverse_f32", Lm, Linv, batch, n);
+  if (rc) return rc;
+  if (batch == 0 || n == 0) return 0;
+  const size_t need = (size_t)batch * n * n * sizeof(float);
+  if (!ws || ws_bytes < need) {
+    set_error("chol_tri_inverse_f32: workspace of %zu bytes required, got %zu", need, ws_bytes);
+    return CHOL_E_WORKSPACE;
+  }
+  return tri_inverse(Lm, Linv, batch, (int)n, reinterpret_cast<float*>(ws), reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

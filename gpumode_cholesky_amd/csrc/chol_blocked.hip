@@ -64,95 +64,6 @@ __global__ void k_zero_info(int32_t* info, int64_t batch) {
   if (i < batch) info[i] = 0;
 }
 
-// ---------------------------------------------------------------- diag ----
-// One wavefront per matrix: factor L[k0:k0+64, k0:k0+64] (already fully
-// updated) in registers, write it back, and write W = L11^-1 (64x64,
-// row-major, zeros above the diagonal) to winv[b].
-// Rows past n are padded with the identity, so the partial last block needs
-// no special case.
-__global__ __launch_bounds__(kWave) void k_diag_factor(Batch p, int k0, float* __restrict__ winv) {
-  constexpr int NB = kNB;
-  constexpr int PITCH = NB + 4;
-  __shared__ __align__(16) float Ls[NB * PITCH];
-  __shared__ __align__(16) float cb[NB];
-  const int r = threadIdx.x;
-  const int64_t b = blockIdx.x;
-  const int nb = min(NB, p.n - k0);
-  float* Lb = p.L + b * p.bs;
-  float* row = Lb + (int64_t)(k0 + (r < nb ? r : 0)) * p.ld + k0;
-
-  float a[NB];
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    float v;
-    if (r < nb)
-      v = (j <= r) ? row[j] : 0.f;
-    else
-      v = (j == r) ? 1.f : 0.f;
-    a[j] = v;
-  }
-
-  int32_t bad = 0;
-#pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    cb[r] = a[k];
-    wave_lds_sync();
-    float c[NB];
-#pragma unroll
-    for (int q = (k / 4) * 4; q < NB; q += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(cb + q);
-      c[q] = v.x;
-      c[q + 1] = v.y;
-      c[q + 2] = v.z;
-      c[q + 3] = v.w;
-    }
-    const float piv = c[k];
-    if (bad == 0 && !(piv > 0.f)) bad = k + 1;
-    const float s = sqrtf(piv);
-    const float is = 1.0f / s;
-    const float lk = a[k] * is;
-    const float t = (r > k) ? lk * is : 0.f;
-#pragma unroll
-    for (int j = k + 1; j < NB; ++j) a[j] = fmaf(-t, c[j], a[j]);
-    a[k] = (r > k) ? lk : ((r == k) ? s : 0.f);
-    wave_lds_sync();
-  }
-
-  if (r < nb) {
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (j <= r) row[j] = a[j];
-  }
-  if (p.info && r == 0 && bad != 0 && bad <= nb) {
-    if (p.info[b] == 0) p.info[b] = k0 + bad;
-  }
-
-  // Inverse: stage L11 in LDS; lane c forward-substitutes column c of W.
-#pragma unroll
-  for (int j = 0; j < NB; j += 4)
-    *reinterpret_cast<float4*>(&Ls[r * PITCH + j]) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
-  __syncthreads();
-  const int c = r;
-  float w[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i) {
-    float s = (i == c) ? 1.f : 0.f;
-    const float* Li = &Ls[i * PITCH];
-#pragma unroll
-    for (int q = 0; q < i; q += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(Li + q);
-      s = fmaf(-v.x, w[q], s);
-      if (q + 1 < i) s = fmaf(-v.y, w[q + 1], s);
-      if (q + 2 < i) s = fmaf(-v.z, w[q + 2], s);
-      if (q + 3 < i) s = fmaf(-v.w, w[q + 3], s);
-    }
-    w[i] = s / Li[i];
-  }
-  float* Wb = winv + b * (NB * NB);
-#pragma unroll
-  for (int i = 0; i < NB; ++i) Wb[i * NB + c] = w[i];
-}
-
 }  // namespace
 
 int launch_zero_info(int32_t* info, int64_t batch, hipStream_t s) {
@@ -174,9 +85,5 @@ int launch_copy_lower(const Batch& p, hipStream_t s) {
   return launch_status("k_copy_lower");
 }
 
-int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s) {
-  hipLaunchKernelGGL(k_diag_factor, dim3((unsigned)p.batch), dim3(kWave), 0, s, p, k0, winv);
-  return launch_status("k_diag_factor");
-}
 
 }  // namespace chol

@@ -1,0 +1,364 @@
+// chol_bwd.hip — triangular inverse and the adjoint (backward) of the
+// factorisation, on FP32 matrix cores.
+//
+//   tri_inverse : X = L^-1 by recursive doubling.  64x64 diagonal blocks are
+//                 inverted by one wavefront each (lane = column, forward
+//                 substitution); then for h = 64, 128, ... each pair of
+//                 h-blocks [A 0; B C] gets its off-diagonal block
+//                 -C^-1 (B A^-1) from two batched GEMMs.
+//   backward    : Abar = L^-T S L^-1 with S = 1/2 (Phi(P) + Phi(P)^T),
+//                 P = L^T Lbar, Phi = lower triangle with halved diagonal.
+//                 Three GEMMs (P, T = L^-T S, Abar = T L^-1) and two
+//                 elementwise passes; L^-1 is recomputed unless given.
+//
+// The reference has no backward and reaches its block inverses through
+// Triton leaves + cuBLAS bmm trees (_exp057_tri_inv_leaf32_kernel and
+// _exp061_leaf_inverse, submission.py:3434-3466, :3562-3603); this is the
+// same recursive-doubling identity written as gfx950 kernels.  The adjoint
+// formula is the one torch's autograd uses for torch.linalg.cholesky.
+#include "chol_common.h"
+
+namespace chol {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int GT = 64;       // GEMM tile (M and N)
+constexpr int GKC = 32;      // K chunk
+constexpr int GKP = GKC + 4; // LDS row pitch
+constexpr int kThreads = 256;
+
+// Batched C = alpha * Aop Bop^T + beta * C over an (M x N) output, K deep.
+//   Aop[m][k] = A[m*lda + k] (A_KC) or A[k*lda + m] (!A_KC)
+//   Bop[n][k] = B[n*ldb + k] (B_KC) or B[k*ldb + n] (!B_KC)
+// Batch index z = blockIdx.z splits as (z / nb2, z % nb2) with separate
+// strides, so one launch covers matrices x sub-blocks.
+struct Gemm {
+  const float* A;
+  const float* B;
+  float* C;
+  int64_t lda, ldb, ldc;
+  int M, N, K;
+  float alpha, beta;
+  int64_t sA1, sA2, sB1, sB2, sC1, sC2;
+  int nb2;
+};
+
+template <bool KC_CONTIG>
+__device__ __forceinline__ void gload(float (&r)[GT * GKC / kThreads], const float* base, int64_t ld, int row0,
+                                      int rlim, int k0, int klim, int tid) {
+  // Loads a GT x GKC tile of op(X)[row][k] into registers, 8 floats per thread.
+  constexpr int PER = GT * GKC / kThreads;  // 8
+  if constexpr (KC_CONTIG) {
+    // thread: row = tid / 4, k = (tid % 4) * 8 .. +7
+    const int gr = row0 + (tid >> 2);
+    const int kk = (tid & 3) * 8;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int gk = k0 + kk + u;
+      r[u] = (gr < rlim && gk < klim) ? base[(int64_t)gr * ld + gk] : 0.f;
+    }
+  } else {
+    // thread: k = tid / 8, rows (tid % 8) * 8 .. +7 (contiguous in memory)
+    const int kk = tid >> 3;
+    const int rr = (tid & 7) * 8;
+    const int gk = k0 + kk;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int gr = row0 + rr + u;
+      r[u] = (gr < rlim && gk < klim) ? base[(int64_t)gk * ld + gr] : 0.f;
+    }
+  }
+}
+
+template <bool KC_CONTIG>
+__device__ __forceinline__ void gstore(float* lds, const float (&r)[GT * GKC / kThreads], int tid) {
+  constexpr int PER = GT * GKC / kThreads;
+  if constexpr (KC_CONTIG) {
+    const int rr = tid >> 2;
+    const int kk = (tid & 3) * 8;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) lds[rr * GKP + kk + u] = r[u];
+  } else {
+    const int kk = tid >> 3;
+    const int rr = (tid & 7) * 8;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) lds[(rr + u) * GKP + kk] = r[u];
+  }
+}
+
+template <bool A_KC, bool B_KC>
+__global__ __launch_bounds__(kThreads) void k_gemm(Gemm g) {
+  __shared__ __align__(16) float As[2][GT * GKP];
+  __shared__ __align__(16) float Bs[2][GT * GKP];
+  const int z = blockIdx.z;
+  const int z1 = z / g.nb2, z2 = z - (z / g.nb2) * g.nb2;
+  const float* A = g.A + z1 * g.sA1 + z2 * g.sA2;
+  const float* B = g.B + z1 * g.sB1 + z2 * g.sB2;
+  float* C = g.C + z1 * g.sC1 + z2 * g.sC2;
+  const int m0 = blockIdx.y * GT;
+  const int n0 = blockIdx.x * GT;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * 32;
+  const int wn = (wave & 1) * 32;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  float ra[GT * GKC / kThreads], rb[GT * GKC / kThreads];
+  gload<A_KC>(ra, A, g.lda, m0, g.M, 0, g.K, tid);
+  gload<B_KC>(rb, B, g.ldb, n0, g.N, 0, g.K, tid);
+  gstore<A_KC>(As[0], ra, tid);
+  gstore<B_KC>(Bs[0], rb, tid);
+  __syncthreads();
+  const int nch = (g.K + GKC - 1) / GKC;
+  for (int ch = 0; ch < nch; ++ch) {
+    const int cur = ch & 1;
+    const bool more = ch + 1 < nch;
+    if (more) {
+      gload<A_KC>(ra, A, g.lda, m0, g.M, (ch + 1) * GKC, g.K, tid);
+      gload<B_KC>(rb, B, g.ldb, n0, g.N, (ch + 1) * GKC, g.K, tid);
+    }
+#pragma unroll
+    for (int ks = 0; ks < GKC; ks += 16) {
+      float4 fa[2], fb[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        fa[a] = *reinterpret_cast<const float4*>(&As[cur][(wm + a * 16 + li) * GKP + ks + 4 * lg]);
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+        fb[b] = *reinterpret_cast<const float4*>(&Bs[cur][(wn + b * 16 + li) * GKP + ks + 4 * lg]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].x, fb[b].x, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].y, fb[b].y, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].z, fb[b].z, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].w, fb[b].w, acc[a][b], 0, 0, 0);
+        }
+    }
+    if (more) {
+      gstore<A_KC>(As[cur ^ 1], ra, tid);
+      gstore<B_KC>(Bs[cur ^ 1], rb, tid);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = n0 + wn + b * 16 + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = m0 + wm + a * 16 + 4 * lg + v;
+        if (i < g.M && j < g.N) {
+          float* dst = C + (int64_t)i * g.ldc + j;
+          const float r = g.alpha * acc[a][b][v];
+          *dst = (g.beta == 0.f) ? r : fmaf(g.beta, *dst, r);
+        }
+      }
+    }
+}
+
+int launch_gemm(const Gemm& g, bool a_kc, bool b_kc, int batch, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || batch <= 0) return 0;
+  if (batch > 65535) {
+    set_error("launch_gemm: %d batched products exceed the grid z limit", batch);
+    return CHOL_E_ARG;
+  }
+  dim3 grid((unsigned)((g.N + GT - 1) / GT), (unsigned)((g.M + GT - 1) / GT), (unsigned)batch);
+  if (a_kc && b_kc)
+    hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(kThreads), 0, s, g);
+  else if (a_kc)
+    hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(kThreads), 0, s, g);
+  else if (b_kc)
+    hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(kThreads), 0, s, g);
+  else
+    hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(kThreads), 0, s, g);
+  return launch_status("k_gemm");
+}
+
+// Inverse of the 64x64 diagonal blocks of L (rows/cols past n: identity),
+// one wavefront per (matrix, block); lane c forward-substitutes column c.
+// Writes the blocks into X (zeros above the diagonal inside each block).
+__global__ __launch_bounds__(64) void k_leaf_inverse(const float* __restrict__ L, float* __restrict__ X, int n,
+                                                     int nblk) {
+  constexpr int NB = 64, P = NB + 4;
+  __shared__ __align__(16) float Ls[NB * P];
+  const int c = threadIdx.x;
+  const int64_t b = blockIdx.x / nblk;
+  const int k0 = (blockIdx.x % nblk) * NB;
+  const int nb = min(NB, n - k0);
+  const float* Lb = L + b * (int64_t)n * n;
+  float* Xb = X + b * (int64_t)n * n;
+  // stage the block (row r = lane), identity past nb
+  for (int r = 0; r < NB; ++r) {
+    float v;
+    if (r < nb)
+      v = (c <= r && c < nb) ? Lb[(int64_t)(k0 + r) * n + k0 + c] : 0.f;
+    else
+      v = (c == r) ? 1.f : 0.f;
+    Ls[r * P + c] = v;
+  }
+  __syncthreads();
+  float w[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    float s = (i == c) ? 1.f : 0.f;
+    const float* Li = &Ls[i * P];
+#pragma unroll
+    for (int q = 0; q < i; q += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(Li + q);
+      s = fmaf(-v.x, w[q], s);
+      if (q + 1 < i) s = fmaf(-v.y, w[q + 1], s);
+      if (q + 2 < i) s = fmaf(-v.z, w[q + 2], s);
+      if (q + 3 < i) s = fmaf(-v.w, w[q + 3], s);
+    }
+    w[i] = s * __builtin_amdgcn_rcpf(Li[i]);
+  }
+  if (c < nb) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if (i < nb) Xb[(int64_t)(k0 + i) * n + k0 + c] = w[i];
+  }
+}
+
+__global__ void k_fill_zero(float* __restrict__ p, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.f;
+}
+
+// S = 1/2 (Phi(P) + Phi(P)^T) from the lower triangle of P, in place
+// (P becomes the full symmetric S).
+__global__ void k_sym_phi(float* __restrict__ P, int n, int64_t total) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / ((int64_t)n * n);
+    const int64_t r = e - b * (int64_t)n * n;
+    const int i = (int)(r / n), j = (int)(r % n);
+    if (j > i) continue;
+    float* M = P + b * (int64_t)n * n;
+    const float v = 0.5f * M[(int64_t)i * n + j];
+    M[(int64_t)i * n + j] = v;
+    if (j < i) M[(int64_t)j * n + i] = v;
+  }
+}
+
+// Abar = 1/2 (Abar + Abar^T): removes the rounding asymmetry of the two
+// products (each pair is read and written by one thread).
+__global__ void k_symmetrize(float* __restrict__ X, int n, int64_t total) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / ((int64_t)n * n);
+    const int64_t r = e - b * (int64_t)n * n;
+    const int i = (int)(r / n), j = (int)(r % n);
+    if (j >= i) continue;
+    float* M = X + b * (int64_t)n * n;
+    const float v = 0.5f * (M[(int64_t)i * n + j] + M[(int64_t)j * n + i]);
+    M[(int64_t)i * n + j] = v;
+    M[(int64_t)j * n + i] = v;
+  }
+}
+
+unsigned grid1d(int64_t total) {
+  int64_t g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace
+
+// X = L^-1 for a contiguous batch; ws holds one n x n scratch per matrix.
+int tri_inverse(const float* L, float* X, int64_t batch, int n, float* ws, hipStream_t s) {
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(k_fill_zero, dim3(grid1d(batch * nn)), dim3(256), 0, s, X, batch * nn);
+  int rc = launch_status("k_fill_zero");
+  if (rc) return rc;
+  const int nblk = (n + 63) / 64;
+  hipLaunchKernelGGL(k_leaf_inverse, dim3((unsigned)(batch * nblk)), dim3(64), 0, s, L, X, n, nblk);
+  if ((rc = launch_status("k_leaf_inverse"))) return rc;
+  for (int h = 64; h < n; h *= 2) {
+    // pairs [2ih, 2ih+2h): A = [2ih, 2ih+h), C = [2ih+h, min(2ih+2h, n))
+    const int pairs = (n + 2 * h - 1) / (2 * h);
+    for (int pass = 0; pass < 2; ++pass) {
+      // Full pairs in one batched launch; the last pair may be partial.
+      const int full = (n / (2 * h));
+      const int rest = pairs - full;
+      for (int part = 0; part < 2; ++part) {
+        const int cnt = part == 0 ? full : rest;
+        if (cnt <= 0) continue;
+        const int i0 = part == 0 ? 0 : full;
+        const int a0 = 2 * i0 * h;          // first row/col of A
+        const int c0 = a0 + h;              // first row/col of C
+        if (c0 >= n) continue;
+        const int hc = (n - c0) < h ? (n - c0) : h;  // rows of C
+        Gemm g{};
+        g.nb2 = cnt;
+        if (pass == 0) {
+          // ws[C rows, A cols] = L[C, A] * X[A, A]   (B^T A^-1 ... as NN)
+          g.A = L + (int64_t)c0 * n + a0;  g.lda = n;  // Aop[m][k] = L[c0+m][a0+k]
+          g.B = X + (int64_t)a0 * n + a0;  g.ldb = n;  // Bop[j][k] = X[a0+k][a0+j]
+          g.C = ws + (int64_t)c0 * n + a0; g.ldc = n;
+          g.M = hc; g.N = h; g.K = h;
+          g.alpha = 1.f; g.beta = 0.f;
+        } else {
+          // X[C, A] = -X[C, C] * ws[C, A]
+          g.A = X + (int64_t)c0 * n + c0;  g.lda = n;  // Aop[m][k] = X[c0+m][c0+k]
+          g.B = ws + (int64_t)c0 * n + a0; g.ldb = n;  // Bop[j][k] = ws[c0+k][a0+j]
+          g.C = X + (int64_t)c0 * n + a0;  g.ldc = n;
+          g.M = hc; g.N = h; g.K = hc;
+          g.alpha = -1.f; g.beta = 0.f;
+        }
+        g.sA1 = g.sB1 = g.sC1 = nn;
+        g.sA2 = g.sB2 = g.sC2 = (int64_t)2 * h * n + 2 * h;  // next pair along the diagonal
+        if ((rc = launch_gemm(g, true, false, (int)(batch * cnt), s))) return rc;
+      }
+    }
+  }
+  return 0;
+}
+
+// Abar = L^-T S L^-1, S from P = L^T Lbar.  ws: 2 n x n scratch per matrix
+// (+1 more when Linv is recomputed).
+int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int n, const float* Linv_in,
+             float* ws, hipStream_t s) {
+  const int64_t nn = (int64_t)n * n;
+  float* P = ws;                 // batch x n x n
+  float* T = ws + batch * nn;    // batch x n x n
+  const float* Linv = Linv_in;
+  int rc;
+  if (!Linv) {
+    float* Xi = ws + 2 * batch * nn;
+    // the inverse's own scratch reuses T (free until step 3)
+    if ((rc = tri_inverse(L, Xi, batch, n, T, s))) return rc;
+    Linv = Xi;
+  }
+  // 1. P = L^T Lbar:  Aop[i][k] = L[k][i], Bop[j][k] = Lbar[k][j].  For
+  //    i >= j only k >= i >= j contributes (L is lower), so the upper
+  //    triangle of Lbar never reaches the lower triangle of P used below.
+  Gemm g{};
+  g.nb2 = 1;
+  g.sA1 = g.sB1 = g.sC1 = nn;
+  g.A = L; g.lda = n; g.B = Lbar; g.ldb = n; g.C = P; g.ldc = n;
+  g.M = n; g.N = n; g.K = n; g.alpha = 1.f; g.beta = 0.f;
+  if ((rc = launch_gemm(g, false, false, (int)batch, s))) return rc;
+  // 2. S = 1/2 (Phi(P) + Phi(P)^T), in place
+  hipLaunchKernelGGL(k_sym_phi, dim3(grid1d(batch * nn)), dim3(256), 0, s, P, n, batch * nn);
+  if ((rc = launch_status("k_sym_phi"))) return rc;
+  // 3. T = L^-T S:  Aop[i][k] = Linv[k][i], Bop[j][k] = S[j][k]
+  g.A = Linv; g.B = P; g.C = T;
+  if ((rc = launch_gemm(g, false, true, (int)batch, s))) return rc;
+  // 4. Abar = T L^-1:  Aop[i][k] = T[i][k], Bop[j][k] = Linv[k][j]
+  g.A = T; g.B = Linv; g.C = Abar;
+  if ((rc = launch_gemm(g, true, false, (int)batch, s))) return rc;
+  hipLaunchKernelGGL(k_symmetrize, dim3(grid1d(batch * nn)), dim3(256), 0, s, Abar, n, batch * nn);
+  return launch_status("k_symmetrize");
+}
+
+}  // namespace chol

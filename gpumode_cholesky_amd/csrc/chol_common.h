@@ -39,6 +39,46 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Right-looking factorisation of an NP x NP block held one row per lane:
+// a[] is the lane's row r of the block (lower part; the upper part is never
+// read by other lanes), cb the per-block LDS column buffer (NP floats,
+// 16-byte aligned), `writer` whether this lane owns row r (duplicate lanes
+// compute but do not write).  Pivot step k: every row writes its column-k
+// entry, reads back the pivot c[k] and c[k+1..NP) with broadcast reads,
+// and folds a[j] -= (a[k]/c[k]) c[j].  The pivot reciprocal square root is
+// v_rsq_f32 (~1 ulp); L[r][k] = a[k] * rsq(c[k]) for r > k, sqrt(c[k]) =
+// c[k] * rsq(c[k]) for r == k, 0 above the diagonal.  A non-positive pivot
+// yields NaN from there on; the return value is the 1-based first failing
+// pivot (0 if none).
+template <int NP>
+__device__ __forceinline__ int reg_factor(float (&a)[NP], int r, float* cb, bool writer) {
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    if (writer) cb[r] = a[k];
+    wave_lds_sync();
+    float c[NP];
+#pragma unroll
+    for (int q = (k / 4) * 4; q < NP; q += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(cb + q);
+      c[q] = v.x;
+      c[q + 1] = v.y;
+      c[q + 2] = v.z;
+      c[q + 3] = v.w;
+    }
+    const float piv = c[k];
+    if (bad == 0 && !(piv > 0.f)) bad = k + 1;
+    const float is = __builtin_amdgcn_rsqf(piv);
+    const float lk = a[k] * is;
+    const float t = (r > k) ? lk * is : 0.f;
+#pragma unroll
+    for (int j = k + 1; j < NP; ++j) a[j] = fmaf(-t, c[j], a[j]);
+    a[k] = (r > k) ? lk : ((r == k) ? piv * is : 0.f);
+    wave_lds_sync();
+  }
+  return bad;
+}
+
 inline int launch_status(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -61,6 +101,12 @@ int launch_panel(const Batch& p, int k0, const float* winv,
 int launch_syrk(const Batch& p, int col0, int col1, int k0, int kl, bool split,
                 hipStream_t s);
 int launch_zero_info(int32_t* info, int64_t batch, hipStream_t s);
+
+// Contiguous batches (ld = n).  tri_inverse: X = L^-1, ws = batch*n*n floats.
+// backward: Abar from L, Lbar; ws = 2 (Linv given) or 3 batch*n*n floats.
+int tri_inverse(const float* L, float* X, int64_t batch, int n, float* ws, hipStream_t s);
+int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int n, const float* Linv,
+             float* ws, hipStream_t s);
 
 // Diagonal block width of the blocked path.
 constexpr int kNB = 64;

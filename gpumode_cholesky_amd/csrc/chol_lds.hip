@@ -90,31 +90,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_lds(Batch p) {
         a[j + 2] = v.z;
         a[j + 3] = v.w;
       }
-      int bad = 0;
-#pragma unroll
-      for (int k = 0; k < kPB; ++k) {
-        if (lane < kPB) cb[r] = a[k];
-        wave_lds_sync();
-        float c[kPB];
-#pragma unroll
-        for (int q = (k / 4) * 4; q < kPB; q += 4) {
-          const float4 v = *reinterpret_cast<const float4*>(cb + q);
-          c[q] = v.x;
-          c[q + 1] = v.y;
-          c[q + 2] = v.z;
-          c[q + 3] = v.w;
-        }
-        const float piv = c[k];
-        if (bad == 0 && !(piv > 0.f)) bad = k + 1;
-        const float s = sqrtf(piv);
-        const float is = 1.0f / s;
-        const float lk = a[k] * is;
-        const float t = (r > k) ? lk * is : 0.f;
-#pragma unroll
-        for (int j = k + 1; j < kPB; ++j) a[j] = fmaf(-t, c[j], a[j]);
-        a[k] = (r > k) ? lk : ((r == k) ? s : 0.f);
-        wave_lds_sync();
-      }
+      const int bad = reg_factor<kPB>(a, r, cb, lane < kPB);
       if (lane < kPB) {
         // Chunks j <= r only; their entries past r are zeros that land in
         // the row's own padding.
@@ -141,7 +117,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_lds(Batch p) {
           if (q + 2 < i) s = fmaf(-v.z, w[q + 2], s);
           if (q + 3 < i) s = fmaf(-v.w, w[q + 3], s);
         }
-        w[i] = s / Li[i];
+        w[i] = s * __builtin_amdgcn_rcpf(Li[i]);
       }
       if (lane < kPB) {
 #pragma unroll

@@ -64,31 +64,7 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
   }
 
   float* cb = &colbuf[wave][m * PITCH];
-  int32_t bad = 0;
-#pragma unroll
-  for (int k = 0; k < NP; ++k) {
-    cb[r] = a[k];
-    wave_lds_sync();
-    float c[NP];
-#pragma unroll
-    for (int q = (k / 4) * 4; q < NP; q += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(cb + q);
-      c[q] = v.x;
-      c[q + 1] = v.y;
-      c[q + 2] = v.z;
-      c[q + 3] = v.w;
-    }
-    const float piv = c[k];
-    if (bad == 0 && !(piv > 0.f)) bad = k + 1;
-    const float s = sqrtf(piv);
-    const float is = 1.0f / s;
-    const float lk = a[k] * is;
-    const float t = (r > k) ? lk * is : 0.f;
-#pragma unroll
-    for (int j = k + 1; j < NP; ++j) a[j] = fmaf(-t, c[j], a[j]);
-    a[k] = (r > k) ? lk : ((r == k) ? s : 0.f);
-    wave_lds_sync();
-  }
+  const int32_t bad = reg_factor<NP>(a, r, cb, true);
 
   if (!live) return;
   if (p.info && r == 0) p.info[b] = (bad > n) ? 0 : bad;
