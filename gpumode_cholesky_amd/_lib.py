@@ -30,6 +30,7 @@ SIGNATURES = {
     "chol_last_error": (ctypes.c_char_p, []),
     "chol_workspace_bytes": (_sz, [_i64, _i64, _i32]),
     "chol_potrf_f32": (ctypes.c_int, [_p, _p, _i64, _i64, _i64, _i64, _i32, _p, _p, _sz, _p]),
+    "chol_set_tuning": (ctypes.c_int, [_p, _i64]),
     "chol_ragged_workspace_bytes": (_sz, [_p, _i64]),
     "chol_potrf_ragged_f32": (ctypes.c_int, [_p, _p, _p, _p, _i64, _p, _p, _sz, _p]),
     "chol_backward_workspace_bytes": (_sz, [_i64, _i64]),
@@ -68,6 +69,9 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = lib
+    # The autotuner's route / blocking table, when one has been written.
+    from . import autotune
+    autotune.load_table()
     return _lib
 
 

@@ -1,0 +1,117 @@
+"""Autotuner for the route / blocking table of the C library.
+
+The library picks, per (n, batch), either the whole-matrix LDS kernel
+(n <= 256) or the blocked path with an outer block width ``nbo`` (columns per
+trailing SYRK).  This module times the candidates on the GPU, keeps the
+fastest per shape, and writes a JSON table that ``load_table`` (called when
+the library is first loaded) hands to ``chol_set_tuning``.  It replaces the
+reference's hand-maintained per-shape tables (submission.py:2263-2281,
+:3262-3281, :5364-5371).
+
+    python -m gpumode_cholesky_amd.autotune [--out PATH] [--shapes 4x1024,...]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import _lib
+
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+NBO_CANDIDATES = (64, 128, 256, 512, 1024)
+
+
+def set_table(rows) -> None:
+    """Install rows [n_lo, n_hi, batch_lo, batch_hi, nbo] (first match wins)."""
+    arr = np.ascontiguousarray(np.asarray(rows, dtype=np.int64).reshape(-1, 5))
+    _lib.check(_lib.load().chol_set_tuning(arr.ctypes.data if len(arr) else None, len(arr)),
+               "chol_set_tuning")
+
+
+def load_table(path: str = DEFAULT_TABLE) -> list:
+    """Install the table stored at ``path`` (no-op when the file is absent)."""
+    if not os.path.exists(path):
+        return []
+    with open(path) as f:
+        rows = json.load(f)["rows"]
+    set_table(rows)
+    return rows
+
+
+def candidates(n: int):
+    out = [-1] if n <= 256 else []
+    if n > 64:
+        out += [c for c in NBO_CANDIDATES if c <= max(64, n)]
+    return out
+
+
+def time_config(a: torch.Tensor, mode: str, reps: int = 5) -> float:
+    """Median microseconds per call over ``reps`` event-timed calls."""
+    from .cholesky import cholesky
+    from .harness import flush_caches
+
+    cholesky(a, mode=mode)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        flush_caches(a.device)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        cholesky(a, mode=mode)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e3)
+    return float(np.median(ts))
+
+
+def tune(shapes, mode: str = "plain", reps: int = 5, log=print) -> list:
+    """Time every candidate for each (batch, n); return the winning rows."""
+    from .harness import config_input
+
+    rows = []
+    dev = torch.device("cuda", torch.cuda.current_device())
+    for batch, n in shapes:
+        if n <= 64:
+            continue
+        a = config_input(batch, n, 1234 + n, device=dev)
+        best = None
+        for nbo in candidates(n):
+            set_table([[n, n, batch, batch, nbo]])
+            us = time_config(a, mode, reps)
+            log(f"tune {batch}x{n} {mode} nbo={nbo}: {us:.1f} us")
+            if best is None or us < best[1]:
+                best = (nbo, us)
+        rows.append([n, n, batch, batch, best[0]])
+        log(f"tune {batch}x{n} {mode}: best nbo={best[0]} ({best[1]:.1f} us)")
+        del a
+        torch.cuda.empty_cache()
+    set_table([])
+    return rows
+
+
+def main():
+    from .harness import BENCH_SPECS
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_TABLE)
+    ap.add_argument("--shapes", default="")
+    ap.add_argument("--mode", default="plain", choices=["plain", "mixed"])
+    ap.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.shapes:
+        shapes = [tuple(int(x) for x in s.split("x")) for s in args.shapes.split(",") if s]
+    else:
+        shapes = [(s["batch"], s["n"]) for s in BENCH_SPECS]
+    rows = tune(shapes, args.mode, args.reps, log=lambda m: print(m, flush=True))
+    with open(args.out, "w") as f:
+        json.dump({"mode": args.mode, "rows": rows}, f, indent=1)
+    print(f"wrote {len(rows)} rows to {args.out}")
+
+
+if __name__ == "__main__":
+    main()

@@ -10,6 +10,8 @@
 //                    blocks with explicit inverses, MFMA panel and SYRK.
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "chol_common.h"
 
@@ -26,24 +28,42 @@ void set_error(const char* fmt, ...) {
 
 void clear_error() { g_err[0] = '\0'; }
 
-// Outer block width of the blocked path (columns per trailing SYRK).
-static int outer_block(int64_t n) { return n <= 256 ? 256 : 256; }
-
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Largest order of the whole-matrix-in-LDS kernel.
 constexpr int64_t kLdsMaxN = 256;
 
+// Tuning table (chol_set_tuning): rows {n_lo, n_hi, batch_lo, batch_hi, nbo}.
+// nbo > 0: blocked path with that outer block width (a multiple of 64);
+// nbo = -1: the whole-matrix LDS kernel (n <= 256 only).  First match wins.
+struct TuneRow {
+  int64_t n_lo, n_hi, b_lo, b_hi, nbo;
+};
+static std::mutex g_tune_mu;
+static std::vector<TuneRow> g_tune;
+
+// Default when no row matches: LDS kernel up to 256, then outer blocks of
+// 256 columns (512 from n = 12288, where the trailing matrix outgrows the
+// 256 MiB Infinity Cache and the SYRK's C traffic dominates).
+static int64_t pick_nbo(int64_t n, int64_t batch) {
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const TuneRow& r : g_tune)
+      if (n >= r.n_lo && n <= r.n_hi && batch >= r.b_lo && batch <= r.b_hi) return r.nbo;
+  }
+  if (n <= kLdsMaxN) return -1;
+  return n >= 12288 ? 512 : 256;
+}
+
 static size_t potrf_ws(int64_t batch, int64_t n) {
-  if (n <= kLdsMaxN || batch <= 0) return 0;
+  if (n <= 64 || batch <= 0) return 0;
   return align256((size_t)batch * kNB * kNB * sizeof(float));
 }
 
-static int blocked_potrf(const Batch& p, float* winv, bool split, hipStream_t s) {
+static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipStream_t s) {
   int rc = launch_copy_lower(p, s);
   if (rc) return rc;
   const int n = p.n;
-  const int nbo = outer_block(n);
   for (int K0 = 0; K0 < n; K0 += nbo) {
     const int W = (n - K0) < nbo ? (n - K0) : nbo;
     for (int k0 = K0; k0 < K0 + W; k0 += kNB) {
@@ -109,7 +129,9 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n, int64_t b
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   Batch p{A, L, batch, (int)n, batch > 1 ? batch_stride : n * ld, ld, info};
   if (n <= 64) return launch_small(p, s);
-  if (n <= kLdsMaxN) return launch_lds(p, s);
+  int64_t nbo = pick_nbo(n, batch);
+  if (nbo < 0 && n <= kLdsMaxN) return launch_lds(p, s);
+  if (nbo < kNB || nbo % kNB != 0) nbo = 256;
   int rc = launch_zero_info(info, batch, s);
   if (rc) return rc;
   // The blocked kernels put the batch on grid.y (at most 65535 blocks).
@@ -120,8 +142,30 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n, int64_t b
     q.A = A + b0 * p.bs;
     q.L = L + b0 * p.bs;
     q.info = info ? info + b0 : nullptr;
-    if ((rc = blocked_potrf(q, reinterpret_cast<float*>(ws), mode == CHOL_MODE_MIXED, s))) return rc;
+    if ((rc = blocked_potrf(q, reinterpret_cast<float*>(ws), mode == CHOL_MODE_MIXED, (int)nbo, s))) return rc;
   }
+  return 0;
+}
+
+int chol_set_tuning(const int64_t* table, int64_t rows) {
+  clear_error();
+  if (rows < 0 || (rows > 0 && !table)) {
+    set_error("chol_set_tuning: bad table");
+    return CHOL_E_ARG;
+  }
+  std::vector<TuneRow> t;
+  for (int64_t i = 0; i < rows; ++i) {
+    const int64_t* r = table + 5 * i;
+    const int64_t nbo = r[4];
+    if (!(nbo == -1 || (nbo >= kNB && nbo % kNB == 0 && nbo <= 4096)) || (nbo == -1 && r[1] > kLdsMaxN)) {
+      set_error("chol_set_tuning: row %lld: nbo=%lld must be -1 (n <= %lld) or a multiple of %d", (long long)i,
+                (long long)nbo, (long long)kLdsMaxN, kNB);
+      return CHOL_E_ARG;
+    }
+    t.push_back(TuneRow{r[0], r[1], r[2], r[3], nbo});
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.swap(t);
   return 0;
 }
 

@@ -46,8 +46,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // compute but do not write).  Pivot step k: every row writes its column-k
 // entry, reads back the pivot c[k] and c[k+1..NP) with broadcast reads,
 // and folds a[j] -= (a[k]/c[k]) c[j].  The pivot reciprocal square root is
-// v_rsq_f32 (~1 ulp); L[r][k] = a[k] * rsq(c[k]) for r > k, sqrt(c[k]) =
-// c[k] * rsq(c[k]) for r == k, 0 above the diagonal.  A non-positive pivot
+// v_rsq_f32 (~1 ulp); L[r][k] = a[k] * rsq(c[k]) for r > k, sqrt(c[k]) from
+// c[k] * rsq(c[k]) plus one Newton correction for r == k (exact for perfect
+// squares), 0 above the diagonal.  A non-positive pivot
 // yields NaN from there on; the return value is the 1-based first failing
 // pivot (0 if none).
 template <int NP>
@@ -73,7 +74,10 @@ __device__ __forceinline__ int reg_factor(float (&a)[NP], int r, float* cb, bool
     const float t = (r > k) ? lk * is : 0.f;
 #pragma unroll
     for (int j = k + 1; j < NP; ++j) a[j] = fmaf(-t, c[j], a[j]);
-    a[k] = (r > k) ? lk : ((r == k) ? piv * is : 0.f);
+    // diagonal: piv * rsq(piv), one Newton step on the residual piv - s^2
+    const float s0 = piv * is;
+    const float s = fmaf(0.5f * is, fmaf(-s0, s0, piv), s0);
+    a[k] = (r > k) ? lk : ((r == k) ? s : 0.f);
     wave_lds_sync();
   }
   return bad;

@@ -70,6 +70,56 @@ __device__ __forceinline__ void store_rows(float* __restrict__ lds, const float4
   }
 }
 
+// Write back a wave's FM x FN blocks of 16x16 accumulators (lane holds
+// D[4*lg + v][li] of each block) at rows r0.., columns c0..:
+//   MODE 0: C -= D where i < ilim, j < jlim, j <= i (lower trailing part);
+//   MODE 1: C  = D where i < ilim.
+// Every C load is issued unconditionally (a masked element reads C[0][0],
+// always valid) so the loads stay in flight together; only the stores are
+// masked.
+template <int FM, int FN, int MODE>
+__device__ __forceinline__ void epilogue(const floatx4 (&acc)[FM][FN], float* C, int64_t ldc, int r0, int c0,
+                                         int ilim, int jlim, int li, int lg) {
+  int64_t off[FM][FN][4];
+  bool ok[FM][FN][4];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = r0 + a * 16 + 4 * lg + v;
+        const int j = c0 + b * 16 + li;
+        const bool m = (MODE == 0) ? (i < ilim && j < jlim && j <= i) : (i < ilim);
+        ok[a][b][v] = m;
+        off[a][b][v] = m ? (int64_t)i * ldc + j : 0;
+      }
+  if constexpr (MODE == 0) {
+    float old[FM][FN][4];
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) old[a][b][v] = C[off[a][b][v]];
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (ok[a][b][v]) C[off[a][b][v]] = old[a][b][v] - acc[a][b][v];
+  } else {
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (ok[a][b][v]) C[off[a][b][v]] = acc[a][b][v];
+  }
+}
+
 // MODE 0: C -= A B^T on the lower part (i >= j), j < jlim, i < ilim.
 // MODE 1: C  = A B^T, i < ilim (panel; A and C alias row-for-row).
 template <int BM, int BN, int MODE, bool VEC>
@@ -150,23 +200,7 @@ __device__ __forceinline__ void gemm_tile(const float* Abase, int64_t lda, int a
   }
 
   // Epilogue: lane holds D[4*lg + v][li] of each 16x16 block.
-#pragma unroll
-  for (int a = 0; a < FM; ++a)
-#pragma unroll
-    for (int b = 0; b < FN; ++b) {
-      const int j = c_col0 + wn + b * 16 + li;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = c_row0 + wm + a * 16 + 4 * lg + v;
-        if (i >= c_ilim) continue;
-        float* dst = Cbase + (int64_t)i * ldc + j;
-        if constexpr (MODE == 0) {
-          if (j < c_jlim && j <= i) *dst -= acc[a][b][v];
-        } else {
-          *dst = acc[a][b][v];
-        }
-      }
-    }
+  epilogue<FM, FN, MODE>(acc, Cbase, ldc, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
 }
 
 // ------------------------------------------------------------ BF16x3 ----
@@ -275,17 +309,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda
     __syncthreads();
   }
 
-#pragma unroll
-  for (int a = 0; a < FM; ++a)
-#pragma unroll
-    for (int b = 0; b < FN; ++b) {
-      const int j = c_col0 + wn + b * 16 + li;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = c_row0 + wm + a * 16 + 4 * lg + v;
-        if (i < c_ilim && j < c_jlim && j <= i) Cbase[(int64_t)i * ldc + j] -= acc[a][b][v];
-      }
-    }
+  epilogue<FM, FN, 0>(acc, Cbase, ldc, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
 }
 
 // Tiles of the lower region {(i, j): col0 <= j < col1, j <= i < n} in 128x128
@@ -301,33 +325,48 @@ __device__ __forceinline__ void decode_tile(int idx, int TI, int& ti, int& tj) {
   ti = t + (idx - (int)((int64_t)t * TI - (int64_t)t * (t - 1) / 2));
 }
 
-constexpr int ST = 128;  // SYRK tile
-
-template <bool VEC, bool SPLIT>
+// SYRK tile T x T (128, or 64 when a launch has too few 128-tiles to fill
+// the GPU); panel row tile PT (same rule).
+template <int T, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void k_syrk_mfma(Batch p, int col0, int col1, int k0, int kl, int TI) {
   int ti, tj;
   decode_tile(blockIdx.x, TI, ti, tj);
   const int64_t b = blockIdx.y;
   float* Lb = p.L + b * p.bs;
-  const int i0 = col0 + ST * ti;
-  const int j0 = col0 + ST * tj;
+  const int i0 = col0 + T * ti;
+  const int j0 = col0 + T * tj;
   if constexpr (SPLIT)
-    gemm_tile_bf16x3<ST, ST, VEC>(Lb, p.ld, i0, p.n, k0, Lb, p.ld, j0, col1, k0, kl, Lb, p.ld, i0, j0, p.n, col1);
+    gemm_tile_bf16x3<T, T, VEC>(Lb, p.ld, i0, p.n, k0, Lb, p.ld, j0, col1, k0, kl, Lb, p.ld, i0, j0, p.n, col1);
   else
-    gemm_tile<ST, ST, 0, VEC>(Lb, p.ld, i0, p.n, k0, Lb, p.ld, j0, col1, k0, kl, Lb, p.ld, i0, j0, p.n, col1);
+    gemm_tile<T, T, 0, VEC>(Lb, p.ld, i0, p.n, k0, Lb, p.ld, j0, col1, k0, kl, Lb, p.ld, i0, j0, p.n, col1);
 }
 
-template <bool VEC>
+template <int PT, bool VEC>
 __global__ __launch_bounds__(kThreads) void k_panel_mfma(Batch p, int k0, const float* __restrict__ winv) {
   const int64_t b = blockIdx.y;
   float* Lb = p.L + b * p.bs;
-  const int i0 = k0 + kNB + 128 * blockIdx.x;
+  const int i0 = k0 + kNB + PT * blockIdx.x;
   const float* Wb = winv + b * (kNB * kNB);
-  gemm_tile<128, kNB, 1, VEC>(Lb, p.ld, i0, p.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, p.ld, i0, k0, p.n, k0 + kNB);
+  gemm_tile<PT, kNB, 1, VEC>(Lb, p.ld, i0, p.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, p.ld, i0, k0, p.n, k0 + kNB);
 }
 
 bool vec_ok(const Batch& p) {
   return (reinterpret_cast<uintptr_t>(p.L) & 15) == 0 && (p.ld % 4) == 0 && (p.bs % 4) == 0;
+}
+
+// Workgroups that keep every CU busy with at least two (256 CUs x 2).
+constexpr int64_t kFillWGs = 512;
+
+template <int T, bool SPLIT>
+void syrk_launch(const Batch& p, int col0, int col1, int k0, int kl, bool vec, hipStream_t s) {
+  const int TI = (p.n - col0 + T - 1) / T;
+  const int TJ = (col1 - col0 + T - 1) / T;
+  const int64_t tiles = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
+  dim3 grid((unsigned)tiles, (unsigned)p.batch);
+  if (vec)
+    hipLaunchKernelGGL((k_syrk_mfma<T, true, SPLIT>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+  else
+    hipLaunchKernelGGL((k_syrk_mfma<T, false, SPLIT>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
 }
 
 }  // namespace
@@ -335,11 +374,21 @@ bool vec_ok(const Batch& p) {
 int launch_panel(const Batch& p, int k0, const float* winv, hipStream_t s) {
   const int rows = p.n - (k0 + kNB);
   if (rows <= 0) return 0;
-  dim3 grid((unsigned)((rows + 127) / 128), (unsigned)p.batch);
-  if (vec_ok(p))
-    hipLaunchKernelGGL(k_panel_mfma<true>, grid, dim3(kThreads), 0, s, p, k0, winv);
-  else
-    hipLaunchKernelGGL(k_panel_mfma<false>, grid, dim3(kThreads), 0, s, p, k0, winv);
+  const bool small = (int64_t)((rows + 127) / 128) * p.batch < kFillWGs;
+  const bool vec = vec_ok(p);
+  if (small) {
+    dim3 grid((unsigned)((rows + 63) / 64), (unsigned)p.batch);
+    if (vec)
+      hipLaunchKernelGGL((k_panel_mfma<64, true>), grid, dim3(kThreads), 0, s, p, k0, winv);
+    else
+      hipLaunchKernelGGL((k_panel_mfma<64, false>), grid, dim3(kThreads), 0, s, p, k0, winv);
+  } else {
+    dim3 grid((unsigned)((rows + 127) / 128), (unsigned)p.batch);
+    if (vec)
+      hipLaunchKernelGGL((k_panel_mfma<128, true>), grid, dim3(kThreads), 0, s, p, k0, winv);
+    else
+      hipLaunchKernelGGL((k_panel_mfma<128, false>), grid, dim3(kThreads), 0, s, p, k0, winv);
+  }
   return launch_status("k_panel_mfma");
 }
 
@@ -349,21 +398,21 @@ int launch_syrk(const Batch& p, int col0, int col1, int k0, int kl, bool split, 
     set_error("launch_syrk: K=%d is not a multiple of %d", kl, KC);
     return CHOL_E_ARG;
   }
-  const int TI = (p.n - col0 + ST - 1) / ST;
-  const int TJ = (col1 - col0 + ST - 1) / ST;
-  const int64_t tiles = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
-  dim3 grid((unsigned)tiles, (unsigned)p.batch);
+  const int TI = (p.n - col0 + 127) / 128;
+  const int TJ = (col1 - col0 + 127) / 128;
+  const int64_t tiles128 = ((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) * p.batch;
+  const bool small = tiles128 < kFillWGs;
   const bool vec = vec_ok(p);
   if (split) {
-    if (vec)
-      hipLaunchKernelGGL((k_syrk_mfma<true, true>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+    if (small)
+      syrk_launch<64, true>(p, col0, col1, k0, kl, vec, s);
     else
-      hipLaunchKernelGGL((k_syrk_mfma<false, true>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+      syrk_launch<128, true>(p, col0, col1, k0, kl, vec, s);
   } else {
-    if (vec)
-      hipLaunchKernelGGL((k_syrk_mfma<true, false>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+    if (small)
+      syrk_launch<64, false>(p, col0, col1, k0, kl, vec, s);
     else
-      hipLaunchKernelGGL((k_syrk_mfma<false, false>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+      syrk_launch<128, false>(p, col0, col1, k0, kl, vec, s);
   }
   return launch_status("k_syrk_mfma");
 }

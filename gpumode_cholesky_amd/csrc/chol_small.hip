@@ -27,31 +27,49 @@ constexpr int kWavesPerBlock = 4;
 template <int NP, bool VEC>
 __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) {
   constexpr int MPW = kWave / NP;  // matrices per wavefront
-  constexpr int PITCH = NP + 4;    // padded column stride (bank spread)
+  constexpr int PITCH = NP + 4;    // padded row / column stride (bank spread)
   __shared__ __align__(16) float colbuf[kWavesPerBlock][MPW * PITCH];
+  // VEC: per-wave staging of its MPW matrices, one padded row per lane.
+  __shared__ __align__(16) float stage[VEC ? kWavesPerBlock : 1][VEC ? kWave * PITCH : 4];
 
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int m = lane / NP;
   const int r = lane - m * NP;
-  const int64_t b = ((int64_t)blockIdx.x * kWavesPerBlock + wave) * MPW + m;
+  const int64_t b0 = ((int64_t)blockIdx.x * kWavesPerBlock + wave) * MPW;  // first matrix of the wave
+  const int64_t b = b0 + m;
   const bool live = b < p.batch;
   const int n = p.n;
 
   float a[NP];
-  const float* src = p.A + (live ? b : 0) * p.bs + (int64_t)(r < n ? r : 0) * p.ld;
   if constexpr (VEC) {
-    // n == NP and every row 16-byte aligned: float4 loads of the lower part.
+    // n == NP, contiguous batch (ld == n, stride n*n), 16-byte aligned: the
+    // wave's MPW matrices are one contiguous run of 64*NP floats.  Lanes read
+    // consecutive float4s (coalesced), skipping chunks above the diagonal,
+    // and the transpose to one row per lane goes through LDS.
+    float* st = stage[wave];
+    const float* src = p.A + b0 * (int64_t)NP * NP;
+#pragma unroll
+    for (int u = 0; u < NP / 4; ++u) {
+      const int e = (u * kWave + lane) * 4;  // element within the run
+      const int row = e / NP;                // 0 .. 63 (matrix-major rows)
+      const int col = e % NP;
+      const int rr = row % NP;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (b0 + row / NP < p.batch && col <= rr) v = *reinterpret_cast<const float4*>(src + e);
+      *reinterpret_cast<float4*>(st + row * PITCH + col) = v;
+    }
+    wave_lds_sync();
 #pragma unroll
     for (int j = 0; j < NP; j += 4) {
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (live && j <= r) v = *reinterpret_cast<const float4*>(src + j);
+      const float4 v = *reinterpret_cast<const float4*>(st + lane * PITCH + j);
       a[j] = v.x;
       a[j + 1] = v.y;
       a[j + 2] = v.z;
       a[j + 3] = v.w;
     }
   } else {
+    const float* src = p.A + (live ? b : 0) * p.bs + (int64_t)(r < n ? r : 0) * p.ld;
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
       float v;
@@ -65,16 +83,28 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
 
   float* cb = &colbuf[wave][m * PITCH];
   const int32_t bad = reg_factor<NP>(a, r, cb, true);
+  if (live && p.info && r == 0) p.info[b] = bad;
 
-  if (!live) return;
-  if (p.info && r == 0) p.info[b] = (bad > n) ? 0 : bad;
-  if (r >= n) return;
-  float* dst = p.L + b * p.bs + (int64_t)r * p.ld;
   if constexpr (VEC) {
+    // Rows back through LDS, then coalesced float4 stores (zeros above the
+    // diagonal included).
+    float* st = stage[wave];
 #pragma unroll
     for (int j = 0; j < NP; j += 4)
-      *reinterpret_cast<float4*>(dst + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+      *reinterpret_cast<float4*>(st + lane * PITCH + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+    wave_lds_sync();
+    float* dst = p.L + b0 * (int64_t)NP * NP;
+#pragma unroll
+    for (int u = 0; u < NP / 4; ++u) {
+      const int e = (u * kWave + lane) * 4;
+      const int row = e / NP;
+      const int col = e % NP;
+      if (b0 + row / NP < p.batch)
+        *reinterpret_cast<float4*>(dst + e) = *reinterpret_cast<const float4*>(st + row * PITCH + col);
+    }
   } else {
+    if (!live || r >= n) return;
+    float* dst = p.L + b * p.bs + (int64_t)r * p.ld;
 #pragma unroll
     for (int j = 0; j < NP; ++j)
       if (j < n) dst[j] = a[j];
@@ -93,8 +123,9 @@ int launch_np(const Batch& p, hipStream_t s) {
 }  // namespace
 
 int launch_small(const Batch& p, hipStream_t s) {
+  // The coalesced form needs n == NP and a contiguous, 16-byte aligned batch.
   const bool aligned = ((reinterpret_cast<uintptr_t>(p.A) | reinterpret_cast<uintptr_t>(p.L)) & 15) == 0 &&
-                       (p.ld % 4) == 0 && (p.bs % 4) == 0;
+                       p.ld == p.n && (p.batch == 1 || p.bs == (int64_t)p.n * p.n);
   if (p.n <= 16) return launch_np<16, false>(p, s);
   if (p.n <= 32) {
     if (p.n == 32 && aligned) return launch_np<32, true>(p, s);

@@ -80,6 +80,19 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n,
                    int32_t* info, void* ws, size_t ws_bytes, void* stream);
 
 /*
+ * Route / blocking table written by the autotuner: `rows` rows of 5 int64
+ *   {n_lo, n_hi, batch_lo, batch_hi, nbo}   (inclusive ranges)
+ * nbo = outer block width of the blocked path (a multiple of 64, <= 4096),
+ * or -1 to keep the whole-matrix LDS kernel (only for n_hi <= 256).  The
+ * first matching row wins; unmatched shapes use built-in defaults; rows = 0
+ * clears the table.  Replaces the reference's hand-written per-shape tables
+ * (submission.py:2263-2281 _SPLIT32_SHAPES, :3262-3281 _LARGE_CFG,
+ * :5364-5371 _EXP062_SHAPES).  Not thread-safe against concurrent
+ * factorisations that read it (set it once, at start-up).
+ */
+int chol_set_tuning(const int64_t* table, int64_t rows);
+
+/*
  * Ragged batch: `count` independent matrices of mixed orders n[i], each
  * contiguous (ld = n[i]).  The pointer and size arrays live in DEVICE memory;
  * n_host is the same sizes in host memory (used to size the launches).

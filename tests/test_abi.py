@@ -73,7 +73,19 @@ def test_rejects_short_workspace():
     assert "workspace" in _lib.last_error()
     # whole-matrix paths (registers, LDS) need none
     assert lib.chol_workspace_bytes(4096, 32, 0) == 0
-    assert lib.chol_workspace_bytes(64, 256, 0) == 0
+    assert lib.chol_workspace_bytes(1024, 64, 0) == 0
+
+
+def test_tuning_table_validation():
+    import numpy as np
+    lib = _lib.load()
+    ok = np.array([[65, 256, 1, 10**9, -1], [257, 4096, 1, 8, 128]], dtype=np.int64)
+    assert lib.chol_set_tuning(ok.ctypes.data, 2) == 0
+    bad = np.array([[300, 512, 1, 10, -1]], dtype=np.int64)  # LDS route beyond n = 256
+    assert lib.chol_set_tuning(bad.ctypes.data, 1) == _lib.E_ARG
+    bad = np.array([[300, 512, 1, 10, 100]], dtype=np.int64)  # not a multiple of 64
+    assert lib.chol_set_tuning(bad.ctypes.data, 1) == _lib.E_ARG
+    assert lib.chol_set_tuning(None, 0) == 0
 
 
 def test_custom_kernel_refuses_cpu_tensors():
