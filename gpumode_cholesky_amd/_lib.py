@@ -69,9 +69,10 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _lib = lib
-    # The autotuner's route / blocking table, when one has been written.
-    from . import autotune
-    autotune.load_table()
+        # The autotuner's route / blocking table, when one has been written
+        # (once, at first load; set_table re-enters load() after _lib is set).
+        from . import autotune
+        autotune.load_table()
     return _lib
 
 
