@@ -169,19 +169,6 @@ int chol_set_tuning(const int64_t* table, int64_t rows) {
   return 0;
 }
 
-size_t chol_ragged_workspace_bytes(const int64_t* n_host, int64_t count) {
-  (void)n_host;
-  (void)count;
-  return 0;
-}
-
-int chol_potrf_ragged_f32(const float* const* A, float* const* L, const int64_t* n_dev, const int64_t* n_host,
-                          int64_t count, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
-  (void)A; (void)L; (void)n_dev; (void)n_host; (void)count; (void)info; (void)ws; (void)ws_bytes; (void)stream;
-  set_error("chol_potrf_ragged_f32: not available in this build");
-  return CHOL_E_ARG;
-}
-
 size_t chol_backward_workspace_bytes(int64_t batch, int64_t n) {
   if (batch <= 0 || n <= 0) return 0;
   // P, T and (when L^-1 is not supplied) the recomputed inverse.

@@ -25,16 +25,16 @@ namespace {
 // ---------------------------------------------------------------- copy ----
 // L = tril(A) with exact zeros above the diagonal.
 __global__ __launch_bounds__(256) void k_copy_lower(Batch p) {
-  const int n = p.n;
-  const int64_t b = blockIdx.y;
-  const float* src = p.A + b * p.bs;
-  float* dst = p.L + b * p.bs;
+  const Mat M = mat_at(p, blockIdx.y);
+  const int n = M.n;
+  const float* src = M.A;
+  float* dst = M.L;
   const int64_t total = (int64_t)n * n;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(e / n);
     const int j = (int)(e - (int64_t)i * n);
-    dst[(int64_t)i * p.ld + j] = (j <= i) ? src[(int64_t)i * p.ld + j] : 0.f;
+    dst[(int64_t)i * M.ld + j] = (j <= i) ? src[(int64_t)i * M.ld + j] : 0.f;
   }
 }
 
@@ -73,7 +73,7 @@ int launch_zero_info(int32_t* info, int64_t batch, hipStream_t s) {
 }
 
 int launch_copy_lower(const Batch& p, hipStream_t s) {
-  const bool vec = ((reinterpret_cast<uintptr_t>(p.A) | reinterpret_cast<uintptr_t>(p.L)) & 15) == 0 &&
+  const bool vec = !p.Ap && ((reinterpret_cast<uintptr_t>(p.A) | reinterpret_cast<uintptr_t>(p.L)) & 15) == 0 &&
                    (p.ld % 4) == 0 && (p.bs % 4) == 0 && (p.n % 4) == 0;
   const int64_t elems = vec ? (int64_t)p.n * (p.n / 4) : (int64_t)p.n * p.n;
   int64_t gx = (elems + 255) / 256;

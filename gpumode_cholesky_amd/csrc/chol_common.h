@@ -19,6 +19,9 @@ void clear_error();
 constexpr int kWave = 64;
 
 // Arguments of one batched factorisation (row-major, see chol_amd.h).
+// Dense form: matrix b at A + b*bs, order n, leading dimension ld.
+// Ragged form (Ap non-null): matrix b at Ap[b] / Lp[b], order ns[b],
+// contiguous (ld = ns[b]); its info slot is orig[b]; n is the largest order.
 struct Batch {
   const float* A;
   float* L;
@@ -27,7 +30,28 @@ struct Batch {
   int64_t bs;  // batch stride (elements)
   int64_t ld;  // leading dimension (elements)
   int32_t* info;
+  const float* const* Ap = nullptr;
+  float* const* Lp = nullptr;
+  const int32_t* ns = nullptr;
+  const int32_t* orig = nullptr;
 };
+
+// One matrix of a Batch, dense or ragged.
+struct Mat {
+  const float* A;
+  float* L;
+  int n;
+  int64_t ld;
+  int64_t slot;  // index into info
+};
+
+__device__ __forceinline__ Mat mat_at(const Batch& p, int64_t b) {
+  if (p.Ap) {
+    const int n = p.ns[b];
+    return Mat{p.Ap[b], p.Lp[b], n, (int64_t)n, p.orig ? (int64_t)p.orig[b] : b};
+  }
+  return Mat{p.A + b * p.bs, p.L + b * p.bs, p.n, p.ld, b};
+}
 
 // Compiler-level barrier between LDS traffic of lanes of one wavefront.
 // LDS instructions of a wave execute in order, so lanes exchanging values

@@ -50,9 +50,11 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   const int li = lane & 15;
   const int lg = lane >> 4;
   const int64_t b = blockIdx.x;
-  const int nb = min(NB, p.n - k0);
-  float* Lb = p.L + b * p.bs + (int64_t)k0 * p.ld + k0;
-  const bool vec = ((p.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(p.L) & 15) == 0) && ((p.bs & 3) == 0);
+  const Mat M = mat_at(p, b);
+  const int nb = min(NB, M.n - k0);
+  if (nb <= 0) return;  // (ragged) this matrix has no block at k0
+  float* Lb = M.L + (int64_t)k0 * M.ld + k0;
+  const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0);
 
   // ---- 1. stage the block: thread covers rows tid/16 + 16u, cols 4*(tid%16) ----
   const int c4 = (tid & 15) * 4;
@@ -62,7 +64,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (r < nb) {
       if (c4 <= r) {
-        const float* src = Lb + (int64_t)r * p.ld + c4;
+        const float* src = Lb + (int64_t)r * M.ld + c4;
         if (vec && c4 + 3 < nb) {
           v = *reinterpret_cast<const float4*>(src);
         } else {
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 #pragma unroll
     for (int j = 0; j < NB; j += 4)
       *reinterpret_cast<float4*>(&Ls[lane * P + j]) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
-    if (p.info && lane == 0 && bad != 0 && bad <= nb && p.info[b] == 0) p.info[b] = k0 + bad;
+    if (p.info && lane == 0 && bad != 0 && bad <= nb && p.info[M.slot] == 0) p.info[M.slot] = k0 + bad;
   }
   __syncthreads();
 
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     *reinterpret_cast<float4*>(&Wb[r * NB + c4]) = w4;
     if (r < nb && c4 <= r) {
       const float4 v = *reinterpret_cast<const float4*>(&Ls[r * P + c4]);
-      float* dst = Lb + (int64_t)r * p.ld + c4;
+      float* dst = Lb + (int64_t)r * M.ld + c4;
       if (vec && c4 + 3 < nb) {
         *reinterpret_cast<float4*>(dst) = v;
       } else {

@@ -328,29 +328,38 @@ __device__ __forceinline__ void decode_tile(int idx, int TI, int& ti, int& tj) {
 // SYRK tile T x T (128, or 64 when a launch has too few 128-tiles to fill
 // the GPU); panel row tile PT (same rule).
 template <int T, bool VEC, bool SPLIT>
-__global__ __launch_bounds__(kThreads) void k_syrk_mfma(Batch p, int col0, int col1, int k0, int kl, int TI) {
+__global__ __launch_bounds__(kThreads) void k_syrk_mfma(Batch p, int col0, int col1, int k0, int kl) {
+  const int64_t b = blockIdx.y;
+  const Mat M = mat_at(p, b);
+  const int c1 = col1 < 0 ? M.n : col1;  // col1 < 0: up to this matrix's order
+  if (c1 <= col0) return;
+  const int TI = (M.n - col0 + T - 1) / T;
+  const int TJ = (c1 - col0 + T - 1) / T;
+  if ((int64_t)blockIdx.x >= (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) return;
   int ti, tj;
   decode_tile(blockIdx.x, TI, ti, tj);
-  const int64_t b = blockIdx.y;
-  float* Lb = p.L + b * p.bs;
+  float* Lb = M.L;
   const int i0 = col0 + T * ti;
   const int j0 = col0 + T * tj;
   if constexpr (SPLIT)
-    gemm_tile_bf16x3<T, T, VEC>(Lb, p.ld, i0, p.n, k0, Lb, p.ld, j0, col1, k0, kl, Lb, p.ld, i0, j0, p.n, col1);
+    gemm_tile_bf16x3<T, T, VEC>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, kl, Lb, M.ld, i0, j0, M.n, c1);
   else
-    gemm_tile<T, T, 0, VEC>(Lb, p.ld, i0, p.n, k0, Lb, p.ld, j0, col1, k0, kl, Lb, p.ld, i0, j0, p.n, col1);
+    gemm_tile<T, T, 0, VEC>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, kl, Lb, M.ld, i0, j0, M.n, c1);
 }
 
 template <int PT, bool VEC>
 __global__ __launch_bounds__(kThreads) void k_panel_mfma(Batch p, int k0, const float* __restrict__ winv) {
   const int64_t b = blockIdx.y;
-  float* Lb = p.L + b * p.bs;
+  const Mat M = mat_at(p, b);
   const int i0 = k0 + kNB + PT * blockIdx.x;
+  if (i0 >= M.n) return;  // (ragged) rows of a smaller matrix
+  float* Lb = M.L;
   const float* Wb = winv + b * (kNB * kNB);
-  gemm_tile<PT, kNB, 1, VEC>(Lb, p.ld, i0, p.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, p.ld, i0, k0, p.n, k0 + kNB);
+  gemm_tile<PT, kNB, 1, VEC>(Lb, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, M.ld, i0, k0, M.n, k0 + kNB);
 }
 
 bool vec_ok(const Batch& p) {
+  if (p.Ap) return p.ld % 4 == 0;  // ragged: the host sets ld = 4 when every matrix is aligned
   return (reinterpret_cast<uintptr_t>(p.L) & 15) == 0 && (p.ld % 4) == 0 && (p.bs % 4) == 0;
 }
 
@@ -359,14 +368,15 @@ constexpr int64_t kFillWGs = 512;
 
 template <int T, bool SPLIT>
 void syrk_launch(const Batch& p, int col0, int col1, int k0, int kl, bool vec, hipStream_t s) {
+  // grid.x covers the largest matrix (p.n); smaller ragged ones exit early
   const int TI = (p.n - col0 + T - 1) / T;
-  const int TJ = (col1 - col0 + T - 1) / T;
+  const int TJ = ((col1 < 0 ? p.n : col1) - col0 + T - 1) / T;
   const int64_t tiles = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
   dim3 grid((unsigned)tiles, (unsigned)p.batch);
   if (vec)
-    hipLaunchKernelGGL((k_syrk_mfma<T, true, SPLIT>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+    hipLaunchKernelGGL((k_syrk_mfma<T, true, SPLIT>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl);
   else
-    hipLaunchKernelGGL((k_syrk_mfma<T, false, SPLIT>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl, TI);
+    hipLaunchKernelGGL((k_syrk_mfma<T, false, SPLIT>), grid, dim3(kThreads), 0, s, p, col0, col1, k0, kl);
 }
 
 }  // namespace
@@ -393,13 +403,13 @@ int launch_panel(const Batch& p, int k0, const float* winv, hipStream_t s) {
 }
 
 int launch_syrk(const Batch& p, int col0, int col1, int k0, int kl, bool split, hipStream_t s) {
-  if (col1 <= col0 || kl <= 0) return 0;
+  if ((col1 >= 0 && col1 <= col0) || col0 >= p.n || kl <= 0) return 0;
   if (kl % KC != 0) {
     set_error("launch_syrk: K=%d is not a multiple of %d", kl, KC);
     return CHOL_E_ARG;
   }
   const int TI = (p.n - col0 + 127) / 128;
-  const int TJ = (col1 - col0 + 127) / 128;
+  const int TJ = ((col1 < 0 ? p.n : col1) - col0 + 127) / 128;
   const int64_t tiles128 = ((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) * p.batch;
   const bool small = tiles128 < kFillWGs;
   const bool vec = vec_ok(p);

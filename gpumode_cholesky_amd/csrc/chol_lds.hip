@@ -39,7 +39,7 @@ __host__ __device__ constexpr int row_off(int i) {
 }
 
 template <int N>
-__global__ __launch_bounds__(kThreads) void k_potrf_lds(Batch p) {
+__device__ __forceinline__ void lds_factor(const Batch& p, int64_t b) {
   constexpr int kFloats = row_off(N);
   constexpr int WP = kPB + 4;
   __shared__ __align__(16) float S[kFloats];
@@ -50,9 +50,9 @@ __global__ __launch_bounds__(kThreads) void k_potrf_lds(Batch p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int64_t b = blockIdx.x;
-  const int n = p.n;
-  const float* A = p.A + b * p.bs;
+  const Mat M = mat_at(p, b);
+  const int n = M.n;
+  const float* A = M.A;
 
   // ---- load the lower triangle (rows >= n: identity) ----
   for (int i = wave; i < N; i += kThreads / 64) {
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_lds(Batch p) {
     for (int j = lane; j < len; j += 64) {
       float v;
       if (i < n)
-        v = (j <= i) ? A[(int64_t)i * p.ld + j] : 0.f;
+        v = (j <= i) ? A[(int64_t)i * M.ld + j] : 0.f;
       else
         v = (j == i) ? 1.f : 0.f;
       row[j] = v;
@@ -177,16 +177,47 @@ __global__ __launch_bounds__(kThreads) void k_potrf_lds(Batch p) {
   }
 
   // ---- write L (zeros above the diagonal) ----
-  if (p.info && tid == 0) p.info[b] = bad_s;
-  float* Lout = p.L + b * p.bs;
+  if (p.info && tid == 0) p.info[M.slot] = bad_s;
+  float* Lout = M.L;
   for (int i = wave; i < n; i += kThreads / 64) {
     const float* row = S + row_off(i);
-    float* dst = Lout + (int64_t)i * p.ld;
+    float* dst = Lout + (int64_t)i * M.ld;
     for (int j = lane; j < n; j += 64) dst[j] = (j <= i) ? row[j] : 0.f;
   }
 }
 
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_potrf_lds(Batch p) {
+  lds_factor<N>(p, blockIdx.x);
+}
+
+// Persistent form for ragged batches: workgroups pop matrices of the sorted
+// range [first, first + count) from a work queue (an atomic counter) and
+// leave when it is exhausted.
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_potrf_lds_queue(Batch p, int64_t first, int64_t count, int* queue) {
+  __shared__ int slot_s;
+  for (;;) {
+    if (threadIdx.x == 0) slot_s = atomicAdd(queue, 1);
+    __syncthreads();
+    const int slot = slot_s;
+    __syncthreads();
+    if (slot >= count) break;
+    lds_factor<N>(p, first + slot);
+    __syncthreads();
+  }
+}
+
 }  // namespace
+
+int launch_lds_queue(const Batch& p, int64_t first, int64_t count, int nmax, int* queue, int grid, hipStream_t s) {
+  if (count <= 0) return 0;
+  if (nmax <= 128)
+    hipLaunchKernelGGL(k_potrf_lds_queue<128>, dim3((unsigned)grid), dim3(kThreads), 0, s, p, first, count, queue);
+  else
+    hipLaunchKernelGGL(k_potrf_lds_queue<256>, dim3((unsigned)grid), dim3(kThreads), 0, s, p, first, count, queue);
+  return launch_status("k_potrf_lds_queue");
+}
 
 int launch_lds(const Batch& p, hipStream_t s) {
   if (p.n <= 128)

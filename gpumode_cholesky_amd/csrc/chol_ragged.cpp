@@ -1,0 +1,181 @@
+// chol_ragged.cpp — ragged batches: many independent matrices of mixed
+// orders factored together.
+//
+// The host sorts the matrices by order (largest first) and uploads one
+// descriptor table (pointers, orders, original positions) into the
+// workspace.  The sorted list splits into three size classes, each one
+// launch sequence over its contiguous slice:
+//   n <= 64       : persistent wavefront kernel (k_potrf_small_queue), one
+//                   launch per padded order 64/32/16, fed by an atomic work
+//                   queue over the sorted slice;
+//   64 < n <= 256 : persistent workgroup kernel with the matrix in LDS
+//                   (k_potrf_lds_queue), one launch per padded order 256/128;
+//   n > 256       : the blocked path run in lock step over 64-column steps:
+//                   step k0 touches the matrices with n > k0, a prefix of the
+//                   sorted list, and each kernel reads its matrix's order
+//                   from the table (blocks past a smaller matrix exit).
+// The reference factors one fixed (batch, n) per call (custom_kernel,
+// submission.py:4101-4268); this generalises its per-shape dispatch.
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "chol_common.h"
+
+namespace chol {
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Layout {
+  size_t queues, Ap, Lp, ns, orig, winv, total;
+  int64_t nbig;
+};
+
+Layout layout(const int64_t* n, int64_t count) {
+  Layout l{};
+  int64_t nbig = 0;
+  for (int64_t i = 0; i < count; ++i) nbig += n[i] > 256;
+  l.queues = 0;
+  l.Ap = align256(l.queues + 8 * sizeof(int));
+  l.Lp = align256(l.Ap + count * sizeof(void*));
+  l.ns = align256(l.Lp + count * sizeof(void*));
+  l.orig = align256(l.ns + count * sizeof(int32_t));
+  l.winv = align256(l.orig + count * sizeof(int32_t));
+  l.total = align256(l.winv + (size_t)nbig * kNB * kNB * sizeof(float));
+  l.nbig = nbig;
+  return l;
+}
+
+}  // namespace
+
+int launch_small_queue(const Batch& p, int64_t first, int64_t count, int np, int* queue, int grid, hipStream_t s);
+int launch_lds_queue(const Batch& p, int64_t first, int64_t count, int nmax, int* queue, int grid, hipStream_t s);
+
+}  // namespace chol
+
+using namespace chol;
+
+extern "C" {
+
+size_t chol_ragged_workspace_bytes(const int64_t* n_host, int64_t count) {
+  if (!n_host || count <= 0) return 0;
+  return layout(n_host, count).total;
+}
+
+int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, const int64_t* n_host, int64_t count,
+                          int32_t mode, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
+  clear_error();
+  if (count < 0 || (count > 0 && (!A_host || !L_host || !n_host))) {
+    set_error("chol_potrf_ragged_f32: bad count or null host arrays");
+    return CHOL_E_ARG;
+  }
+  if (mode != CHOL_MODE_PLAIN && mode != CHOL_MODE_MIXED) {
+    set_error("chol_potrf_ragged_f32: unknown mode %d", (int)mode);
+    return CHOL_E_MODE;
+  }
+  if (count == 0) return 0;
+  if (count > (int64_t)1 << 30) {
+    set_error("chol_potrf_ragged_f32: count %lld too large", (long long)count);
+    return CHOL_E_ARG;
+  }
+  for (int64_t i = 0; i < count; ++i) {
+    if (n_host[i] < 0 || n_host[i] > 32768 || (n_host[i] > 0 && (!A_host[i] || !L_host[i]))) {
+      set_error("chol_potrf_ragged_f32: matrix %lld: n=%lld or null pointer", (long long)i, (long long)n_host[i]);
+      return CHOL_E_ARG;
+    }
+  }
+  const Layout lay = layout(n_host, count);
+  if (!ws || ws_bytes < lay.total) {
+    set_error("chol_potrf_ragged_f32: workspace of %zu bytes required, got %zu", lay.total, ws_bytes);
+    return CHOL_E_WORKSPACE;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+
+  // Sort by order, largest first (ties keep the caller's order).
+  std::vector<int64_t> ord(count);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return n_host[a] > n_host[b]; });
+  std::vector<char> desc(lay.winv - lay.Ap);
+  auto* Ap = reinterpret_cast<const float**>(desc.data());
+  auto* Lp = reinterpret_cast<float**>(desc.data() + (lay.Lp - lay.Ap));
+  auto* ns = reinterpret_cast<int32_t*>(desc.data() + (lay.ns - lay.Ap));
+  auto* orig = reinterpret_cast<int32_t*>(desc.data() + (lay.orig - lay.Ap));
+  bool aligned = true;
+  for (int64_t i = 0; i < count; ++i) {
+    const int64_t j = ord[i];
+    Ap[i] = A_host[j];
+    Lp[i] = L_host[j];
+    ns[i] = (int32_t)n_host[j];
+    orig[i] = (int32_t)j;
+    aligned = aligned && (ns[i] % 4 == 0) && ((reinterpret_cast<uintptr_t>(A_host[j]) |
+                                                reinterpret_cast<uintptr_t>(L_host[j])) & 15) == 0;
+  }
+  char* w = reinterpret_cast<char*>(ws);
+  // Pageable source: the copy is staged before hipMemcpyAsync returns, so
+  // the host vector may go out of scope afterwards.
+  hipError_t e = hipMemcpyAsync(w + lay.Ap, desc.data(), desc.size(), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    set_error("chol_potrf_ragged_f32: descriptor upload failed: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  int* queues = reinterpret_cast<int*>(w + lay.queues);
+  int rc;
+  if ((rc = launch_zero_info(queues, 8, s))) return rc;
+  if ((rc = launch_zero_info(info, count, s))) return rc;
+
+  Batch base{nullptr, nullptr, 0, 0, 0, aligned ? 4 : 1, info};
+  base.Ap = reinterpret_cast<const float* const*>(w + lay.Ap);
+  base.Lp = reinterpret_cast<float* const*>(w + lay.Lp);
+  base.ns = reinterpret_cast<const int32_t*>(w + lay.ns);
+  base.orig = reinterpret_cast<const int32_t*>(w + lay.orig);
+
+  // Class boundaries in the sorted list (orders decrease along it).
+  auto first_at_most = [&](int64_t lim) {  // first sorted index with n <= lim
+    int64_t i = 0;
+    while (i < count && ns[i] > lim) ++i;
+    return i;
+  };
+  const int64_t c256 = first_at_most(256), c128 = first_at_most(128), c64 = first_at_most(64),
+                c32 = first_at_most(32), c16 = first_at_most(16), cz = first_at_most(0);
+
+  // n > 256: lock-step blocked path over the prefix [0, c256).
+  if (c256 > 0) {
+    Batch q = base;
+    q.batch = c256;
+    q.n = ns[0];
+    float* winv = reinterpret_cast<float*>(w + lay.winv);
+    if ((rc = launch_copy_lower(q, s))) return rc;
+    for (int k0 = 0; k0 < ns[0]; k0 += kNB) {
+      int64_t act = 0, act2 = 0;
+      while (act < c256 && ns[act] > k0) ++act;
+      while (act2 < c256 && ns[act2] > k0 + kNB) ++act2;
+      q.batch = act;
+      q.n = ns[0];
+      if ((rc = launch_diag_factor(q, k0, winv, s))) return rc;
+      if (act2 == 0) break;
+      q.batch = act2;
+      if ((rc = launch_panel(q, k0, winv, s))) return rc;
+      if ((rc = launch_syrk(q, k0 + kNB, -1, k0, kNB, mode == CHOL_MODE_MIXED, s))) return rc;
+    }
+  }
+  // 64 < n <= 256: persistent LDS workgroups.
+  if ((rc = launch_lds_queue(base, c256, c128 - c256, 256, queues + 0, (int)std::min<int64_t>(c128 - c256, 256), s)))
+    return rc;
+  if ((rc = launch_lds_queue(base, c128, c64 - c128, 128, queues + 1, (int)std::min<int64_t>(c64 - c128, 1024), s)))
+    return rc;
+  // n <= 64: persistent wavefronts (4 per workgroup, 64/np matrices each).
+  auto waves = [](int64_t cnt, int np) { return (cnt * np + 63) / 64; };
+  if ((rc = launch_small_queue(base, c64, c32 - c64, 64, queues + 2,
+                               (int)std::min<int64_t>((waves(c32 - c64, 64) + 3) / 4, 512), s)))
+    return rc;
+  if ((rc = launch_small_queue(base, c32, c16 - c32, 32, queues + 3,
+                               (int)std::min<int64_t>((waves(c16 - c32, 32) + 3) / 4, 1024), s)))
+    return rc;
+  if ((rc = launch_small_queue(base, c16, cz - c16, 16, queues + 4,
+                               (int)std::min<int64_t>((waves(cz - c16, 16) + 3) / 4, 1024), s)))
+    return rc;
+  return 0;
+}
+
+}  // extern "C"

@@ -24,6 +24,35 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 
+// Row r of matrix b into a[] (lower part; rows past the matrix's order are
+// the identity, so a padded NP x NP block factors as diag(L, I)).
+template <int NP>
+__device__ __forceinline__ void slot_rows(const Batch& p, int64_t b, bool live, int r, float (&a)[NP]) {
+  const Mat M = mat_at(p, live ? b : 0);
+  const int n = M.n;
+  const float* src = M.A + (int64_t)(r < n ? r : 0) * M.ld;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    float v;
+    if (r < n)
+      v = (live && j <= r) ? src[j] : 0.f;
+    else
+      v = (j == r) ? 1.f : 0.f;
+    a[j] = v;
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ void slot_store(const Batch& p, int64_t b, bool live, int r, const float (&a)[NP]) {
+  if (!live) return;
+  const Mat M = mat_at(p, b);
+  if (r >= M.n) return;
+  float* dst = M.L + (int64_t)r * M.ld;
+#pragma unroll
+  for (int j = 0; j < NP; ++j)
+    if (j < M.n) dst[j] = a[j];
+}
+
 template <int NP, bool VEC>
 __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) {
   constexpr int MPW = kWave / NP;  // matrices per wavefront
@@ -69,21 +98,12 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
       a[j + 3] = v.w;
     }
   } else {
-    const float* src = p.A + (live ? b : 0) * p.bs + (int64_t)(r < n ? r : 0) * p.ld;
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      float v;
-      if (r < n)
-        v = (live && j <= r) ? src[j] : 0.f;
-      else
-        v = (j == r) ? 1.f : 0.f;
-      a[j] = v;
-    }
+    slot_rows<NP>(p, b, live, r, a);
   }
 
   float* cb = &colbuf[wave][m * PITCH];
   const int32_t bad = reg_factor<NP>(a, r, cb, true);
-  if (live && p.info && r == 0) p.info[b] = bad;
+  if (live && p.info && r == 0) p.info[VEC ? b : mat_at(p, b).slot] = bad;
 
   if constexpr (VEC) {
     // Rows back through LDS, then coalesced float4 stores (zeros above the
@@ -103,11 +123,37 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
         *reinterpret_cast<float4*>(dst + e) = *reinterpret_cast<const float4*>(st + row * PITCH + col);
     }
   } else {
-    if (!live || r >= n) return;
-    float* dst = p.L + b * p.bs + (int64_t)r * p.ld;
-#pragma unroll
-    for (int j = 0; j < NP; ++j)
-      if (j < n) dst[j] = a[j];
+    slot_store<NP>(p, b, live, r, a);
+  }
+}
+
+// Persistent form for ragged batches: wavefronts pop slots of MPW matrices
+// from a work queue (an atomic counter) over the sorted range
+// [first, first + count); every wave leaves when the queue is exhausted.
+template <int NP>
+__global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small_queue(Batch p, int64_t first, int64_t count,
+                                                                            int* queue) {
+  constexpr int MPW = kWave / NP;
+  constexpr int PITCH = NP + 4;
+  __shared__ __align__(16) float colbuf[kWavesPerBlock][MPW * PITCH];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int m = lane / NP;
+  const int r = lane - m * NP;
+  float* cb = &colbuf[wave][m * PITCH];
+  for (;;) {
+    int slot = 0;
+    if (lane == 0) slot = atomicAdd(queue, 1);
+    slot = __builtin_amdgcn_readfirstlane(slot);
+    const int64_t idx = (int64_t)slot * MPW;
+    if (idx >= count) break;
+    const int64_t b = first + idx + m;
+    const bool live = idx + m < count;
+    float a[NP];
+    slot_rows<NP>(p, b, live, r, a);
+    const int32_t bad = reg_factor<NP>(a, r, cb, true);
+    if (live && p.info && r == 0) p.info[mat_at(p, b).slot] = bad;
+    slot_store<NP>(p, b, live, r, a);
   }
 }
 
@@ -133,6 +179,19 @@ int launch_small(const Batch& p, hipStream_t s) {
   }
   if (p.n == 64 && aligned) return launch_np<64, true>(p, s);
   return launch_np<64, false>(p, s);
+}
+
+int launch_small_queue(const Batch& p, int64_t first, int64_t count, int np, int* queue, int grid,
+                       hipStream_t s) {
+  if (count <= 0) return 0;
+  const dim3 g((unsigned)grid), blk(kWavesPerBlock * kWave);
+  if (np <= 16)
+    hipLaunchKernelGGL((k_potrf_small_queue<16>), g, blk, 0, s, p, first, count, queue);
+  else if (np <= 32)
+    hipLaunchKernelGGL((k_potrf_small_queue<32>), g, blk, 0, s, p, first, count, queue);
+  else
+    hipLaunchKernelGGL((k_potrf_small_queue<64>), g, blk, 0, s, p, first, count, queue);
+  return launch_status("k_potrf_small_queue");
 }
 
 }  // namespace chol

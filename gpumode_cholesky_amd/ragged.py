@@ -1,0 +1,49 @@
+"""Ragged batches: many matrices of different orders in one call.
+
+``cholesky_ragged(mats)`` factors a list of FP32 SPD device tensors of mixed
+orders through ``chol_potrf_ragged_f32``: the library sorts them by order and
+runs each size class as one launch sequence (persistent work-queue kernels
+for n <= 256, the blocked path in lock step above).  The reference has no
+ragged entry point (it factors one fixed (batch, n) per call,
+submission.py:4101-4268).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .cholesky import _MODES, _raw_stream, _validate
+
+
+def cholesky_ragged(mats, mode: str = "plain", return_info: bool = False):
+    """Factor each tensor of ``mats`` ([n_i, n_i] FP32, one device).
+
+    Returns the list of factors (and an int32 ``info`` tensor, one entry per
+    matrix, when ``return_info``)."""
+    m = _MODES.get(mode)
+    if m is None:
+        raise ValueError(f"mode must be one of {sorted(_MODES)}, got {mode!r}")
+    mats = [a if a.is_contiguous() else a.contiguous() for a in mats]
+    if not mats:
+        return ([], torch.empty(0, dtype=torch.int32)) if return_info else []
+    dev = mats[0].device
+    for a in mats:
+        _validate(a)
+        if a.dim() != 2 or a.device != dev:
+            raise ValueError("every matrix must be a 2-D [n, n] tensor on one device")
+    outs = [torch.empty_like(a) for a in mats]
+    count = len(mats)
+    n_host = np.array([a.shape[-1] for a in mats], dtype=np.int64)
+    a_host = np.array([a.data_ptr() for a in mats], dtype=np.uint64)
+    l_host = np.array([o.data_ptr() for o in outs], dtype=np.uint64)
+    lib = _lib.load()
+    nws = lib.chol_ragged_workspace_bytes(n_host.ctypes.data, count)
+    ws = torch.empty((max(nws, 1),), dtype=torch.uint8, device=dev)
+    info = torch.empty((count,), dtype=torch.int32, device=dev) if return_info else None
+    with torch.cuda.device(dev):
+        rc = lib.chol_potrf_ragged_f32(a_host.ctypes.data, l_host.ctypes.data, n_host.ctypes.data, count, m,
+                                       info.data_ptr() if info is not None else None,
+                                       ws.data_ptr(), nws, _raw_stream(dev.index))
+    _lib.check(rc, "chol_potrf_ragged_f32")
+    return (outs, info) if return_info else outs

@@ -42,7 +42,7 @@ extern "C" {
 
 /* Precision modes. */
 #define CHOL_MODE_PLAIN 0 /* all-FP32 arithmetic                            */
-#define CHOL_MODE_MIXED 1 /* split-BF16 trailing updates + FP64 correction  */
+#define CHOL_MODE_MIXED 1 /* BF16x3 split-product trailing updates, FP32 acc */
 
 /* Version of this ABI (CHOL_ABI_VERSION). */
 int chol_abi_version(void);
@@ -93,17 +93,22 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n,
 int chol_set_tuning(const int64_t* table, int64_t rows);
 
 /*
- * Ragged batch: `count` independent matrices of mixed orders n[i], each
- * contiguous (ld = n[i]).  The pointer and size arrays live in DEVICE memory;
- * n_host is the same sizes in host memory (used to size the launches).
- * Has no counterpart in the reference (which factors one fixed (batch, n) per
- * call); it generalises the per-shape dispatch of submission.py:4101-4268.
+ * Ragged batch: `count` independent matrices of mixed orders n_host[i], each
+ * contiguous (ld = n_host[i]).  A_host[i] / L_host[i] are DEVICE pointers to
+ * matrix i; the three arrays themselves are in HOST memory.  info: nullable
+ * device int32[count], in the caller's order.  The library sorts by order
+ * and runs each size class as one launch sequence: a persistent wavefront
+ * kernel (n <= 64) and a persistent LDS-workgroup kernel (64 < n <= 256),
+ * both fed by atomic work queues over the size-sorted list, and the blocked
+ * path in lock step over the larger matrices.
+ * Has no counterpart in the reference, which factors one fixed (batch, n)
+ * per call; it generalises the per-shape dispatch of submission.py:4101-4268.
  */
 size_t chol_ragged_workspace_bytes(const int64_t* n_host, int64_t count);
-int chol_potrf_ragged_f32(const float* const* A, float* const* L,
-                          const int64_t* n_dev, const int64_t* n_host,
-                          int64_t count, int32_t* info, void* ws,
-                          size_t ws_bytes, void* stream);
+int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host,
+                          const int64_t* n_host, int64_t count, int32_t mode,
+                          int32_t* info, void* ws, size_t ws_bytes,
+                          void* stream);
 
 /*
  * Adjoint of the factorisation: given L (from chol_potrf_f32) and the
