@@ -27,6 +27,7 @@ import os
 import sys
 import time
 
+import numpy as np
 import torch
 
 from gpumode_cholesky_amd import cholesky
@@ -34,6 +35,7 @@ from gpumode_cholesky_amd import harness
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+PMC_TRAFFIC_4096x32 = (8273.1 * 2.0 + 16384.0) * 1024  # bytes per launch
 
 
 def _parse():
@@ -46,6 +48,9 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     ap.add_argument("--detail", default="", help="write per-shape JSON here")
+    ap.add_argument("--no-extras", action="store_true", help="skip ragged / graph / backward measurements")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB: run one device copy of this size first (scales the PMC byte counters)")
     return ap.parse_args()
 
 
@@ -164,6 +169,86 @@ def _time(specs, data, mode, steps, warmup, dev):
     return us
 
 
+def _event_us(fn, reps=5):
+    """Median microseconds of fn() over reps event-timed calls (cache flushed)."""
+    dev = torch.device("cuda", torch.cuda.current_device())
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        harness.flush_caches(dev)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e3)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def extras(dev):
+    """Ragged workload, graph replay and backward measurements (not part of
+    the headline value)."""
+    from gpumode_cholesky_amd.autograd import cholesky_ad
+    from gpumode_cholesky_amd.graphs import GraphedCholesky
+    from gpumode_cholesky_amd.ragged import cholesky_ragged
+
+    out = {}
+    # Ragged: 512 matrices, n log-uniform on [32, 2048], each G G^T/n + I.
+    rng = np.random.default_rng(4096)
+    ns = np.exp(rng.uniform(math.log(32), math.log(2048), 512)).astype(int)
+    mats = [harness.dense_input(1, int(n), 1.0, 4096 + i, device=dev)[0] for i, n in enumerate(ns)]
+    gb = sum(a.numel() * 4 for a in mats) / 1e9
+    rag = {"count": len(mats), "gb": round(gb, 3), "n_min": int(ns.min()), "n_max": int(ns.max())}
+    for mode in ("plain", "mixed"):
+        rag[f"one_call_us_{mode}"] = round(_event_us(lambda: cholesky_ragged(mats, mode=mode), 3), 1)
+    rag["per_matrix_calls_us_plain"] = round(_event_us(lambda: [cholesky(a[None]) for a in mats], 3), 1)
+    out["ragged"] = rag
+    del mats
+    torch.cuda.empty_cache()
+    print("# extras: ragged done", file=sys.stderr, flush=True)
+
+    # Graph replay vs eager calls at the two smallest fixed shapes.
+    gr = {}
+    for b, n in ((4096, 32), (1024, 64)):
+        ins = [harness.config_input(b, n, 31 + i, device=dev) for i in range(8)]
+        g = GraphedCholesky(ins)
+        eager = _event_us(lambda: [cholesky(a) for a in ins], 5) / len(ins)
+        replay = _event_us(lambda: [g.replay(i) for i in range(len(g))], 5) / len(ins)
+        gr[f"{b}x{n}"] = {"eager_us": round(eager, 2), "graph_us": round(replay, 2)}
+        del g, ins
+    out["graph"] = gr
+    print("# extras: graph done", file=sys.stderr, flush=True)
+
+    # Forward + backward (spectral cond 1e2), L^-1 recomputed vs stored.
+    bw = {}
+    for b, n in ((256, 128), (8, 2048)):
+        a = harness.spectral_input(b, n, 1e2, 55, device=dev)
+        gL = torch.tril(torch.randn(b, n, n, device=dev, generator=torch.Generator(device=dev).manual_seed(1)))
+        row = {}
+        for store in (False, True):
+            A = a.clone().requires_grad_(True)
+
+            def step():
+                A.grad = None
+                L = cholesky_ad(A, store_inverse=store)
+                L.backward(gL)
+
+            torch.cuda.synchronize()
+            torch.cuda.reset_peak_memory_stats(dev)
+            base = torch.cuda.memory_allocated(dev)
+            us = _event_us(step, 3)
+            peak = (torch.cuda.max_memory_allocated(dev) - base) / 2**20
+            row["stored" if store else "recomputed"] = {"fwd_bwd_us": round(us, 1), "peak_extra_mib": round(peak, 1)}
+        bw[f"{b}x{n}"] = row
+        del a, gL
+    out["backward"] = bw
+    print("# extras: backward done", file=sys.stderr, flush=True)
+    return out
+
+
 def _geo(xs):
     xs = list(xs)
     return math.exp(sum(math.log(x) for x in xs) / len(xs))
@@ -178,6 +263,13 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     specs = _specs(args.shapes)
+    if args.copy_probe:
+        src = torch.ones((args.copy_probe * 2**20 // 4,), device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads and writes exactly copy_probe MiB
+        torch.cuda.synchronize()
+        del src, dst
     modes = ["plain", "mixed"] if args.mode == "both" else [args.mode]
     data = _inputs(specs, dev)
     resid = {m: _gate(specs, data, m) for m in modes}
@@ -213,9 +305,13 @@ def main():
     r32 = next((r for r in rows if (r["batch"], r["n"]) == (4096, 32)), None)
     if r32 is not None:
         g = r32[head]["GBps"]
+        # traffic: HBM bytes per launch from the two rocprofv3 --pmc passes
+        # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scaled by a 256 MiB copy
+        # probe), recorded in profiles/k_potrf_small_32_4096x32.md.
         roof = {"kernel": "k_potrf_small<32,vec> on 4096x32", "bound": "hbm",
                 "achieved": g, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(g / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(g / HBM_PEAK_GBS, 4), "traffic": PMC_TRAFFIC_4096x32,
+                "algorithmic_bytes": 8 * 4096 * 32 * 32}
     out = {
         "metric": "geomean us over 15 batch x n SPD shapes",
         "value": round(geo[head], 2),
@@ -239,6 +335,8 @@ def main():
     }
     if len(modes) == 2:
         out["mixed_speedup_vs_plain"] = round(geo["plain"] / geo["mixed"], 4)
+    if not args.no_extras:
+        out["extras"] = extras(dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget_s)
     if args.detail:
