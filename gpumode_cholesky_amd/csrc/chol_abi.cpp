@@ -60,10 +60,52 @@ static size_t potrf_ws(int64_t batch, int64_t n) {
   return align256((size_t)batch * kNB * kNB * sizeof(float));
 }
 
+// Per-device side stream (default priority) and the two events that order
+// it against the caller's stream, created on first use and kept for the
+// life of the process.
+struct SideStream {
+  hipStream_t s2 = nullptr;
+  hipEvent_t to_side = nullptr;   // recorded on the caller's stream
+  hipEvent_t from_side = nullptr; // recorded on the side stream
+};
+static std::mutex g_side_mu;
+static SideStream g_side[64];
+
+static int side_stream(SideStream*& out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= 64) {
+    set_error("side stream: hipGetDevice failed (%d)", (int)e);
+    return e != hipSuccess ? (int)e : CHOL_E_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_side_mu);
+  SideStream& r = g_side[dev];
+  if (!r.s2) {
+    if ((e = hipStreamCreateWithFlags(&r.s2, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&r.to_side, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&r.from_side, hipEventDisableTiming)) != hipSuccess) {
+      set_error("side stream: creation failed: %s", hipGetErrorString(e));
+      return (int)e;
+    }
+  }
+  out = &r;
+  return 0;
+}
+
+// Blocked right-looking factorisation with one outer block of look-ahead.
+// Within outer block [K0, K0+W) the 64-wide steps run on the caller's
+// stream s.  The block's trailing update (K = W) is split: the next outer
+// block's columns (narrow, needed next) stay on s; the columns beyond
+// (bulk) go to the side stream, where they overlap the next block's chain
+// of small launches.  A narrow update first waits for the previous bulk
+// update, which wrote the same columns; the caller's stream joins the side
+// stream at the end.
 static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipStream_t s) {
   int rc = launch_copy_lower(p, s);
   if (rc) return rc;
   const int n = p.n;
+  SideStream* side = nullptr;
+  bool bulk_pending = false;
   for (int K0 = 0; K0 < n; K0 += nbo) {
     const int W = (n - K0) < nbo ? (n - K0) : nbo;
     for (int k0 = K0; k0 < K0 + W; k0 += kNB) {
@@ -74,10 +116,30 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
         if ((rc = launch_syrk(p, k0 + kNB, K0 + W, k0, kNB, split, s))) return rc;
       }
     }
-    if (K0 + W < n) {
-      if ((rc = launch_syrk(p, K0 + W, n, K0, W, split, s))) return rc;
+    const int c0 = K0 + W;  // first column after this outer block
+    if (c0 >= n) break;
+    const int c1 = (n - c0) < nbo ? n : c0 + nbo;  // end of the next outer block
+    if (bulk_pending) {
+      if (hipStreamWaitEvent(s, side->from_side, 0) != hipSuccess) return launch_status("hipStreamWaitEvent");
+      bulk_pending = false;
+    }
+    if ((rc = launch_syrk(p, c0, c1, K0, W, split, s))) return rc;  // narrow
+    if (c1 < n) {
+      if (!side && (rc = side_stream(side))) return rc;
+      hipError_t e;
+      if ((e = hipEventRecord(side->to_side, s)) != hipSuccess || (e = hipStreamWaitEvent(side->s2, side->to_side, 0)) != hipSuccess) {
+        set_error("look-ahead: event ordering failed: %s", hipGetErrorString(e));
+        return (int)e;
+      }
+      if ((rc = launch_syrk(p, c1, n, K0, W, split, side->s2))) return rc;  // bulk
+      if ((e = hipEventRecord(side->from_side, side->s2)) != hipSuccess) {
+        set_error("look-ahead: event record failed: %s", hipGetErrorString(e));
+        return (int)e;
+      }
+      bulk_pending = true;
     }
   }
+  if (bulk_pending && hipStreamWaitEvent(s, side->from_side, 0) != hipSuccess) return launch_status("hipStreamWaitEvent");
   return 0;
 }
 
