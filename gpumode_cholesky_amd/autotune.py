@@ -22,7 +22,7 @@ import torch
 from . import _lib
 
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
-NBO_CANDIDATES = (64, 128, 256, 512, 1024)
+NBO_CANDIDATES = (128, 256, 512, 1024, 2048)
 
 
 def set_table(rows) -> None:

@@ -121,7 +121,7 @@ int launch_small(const Batch& p, hipStream_t s);                  // n <= 64
 int launch_lds(const Batch& p, hipStream_t s);                    // n <= 256
 int launch_copy_lower(const Batch& p, hipStream_t s);             // L = tril(A)
 int launch_diag_factor(const Batch& p, int k0, float* winv,
-                       hipStream_t s);                            // 64x64 block
+                       hipStream_t s);                            // kNB x kNB block
 int launch_panel(const Batch& p, int k0, const float* winv,
                  hipStream_t s);                                  // L21 = A21 W^T
 // C -= P P^T over {col0 <= j < col1, j <= i < n}, K = [k0, k0+kl);
@@ -137,6 +137,6 @@ int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int 
              float* ws, hipStream_t s);
 
 // Diagonal block width of the blocked path.
-constexpr int kNB = 64;
+constexpr int kNB = 128;
 
 }  // namespace chol

@@ -1,37 +1,43 @@
-// chol_diag.hip — factor one 64x64 diagonal block per matrix and form its
-// inverse, the serial link of the blocked path (one launch per 64 columns).
+// chol_diag.hip — factor one 128x128 diagonal block per matrix and form its
+// inverse: the serial link of the blocked path (one launch per 128 columns).
 //
-// One workgroup of 4 wavefronts per matrix:
-//   1. all waves stage the (already updated) block into LDS with coalesced
-//      row loads; rows past n are the identity;
-//   2. wavefront 0 factors it in registers, one row per lane (reg_factor);
-//   3. the inverse W = L11^-1 by 16x16 blocks: the 4 wavefronts invert the
-//      four diagonal 16x16 blocks at once (lane = column, forward
-//      substitution); then block rows 1..3 in turn,
-//         W_ij = -W_ii (sum_{k=j}^{i-1} L_ik W_kj),   j < i,
-//      one 16x16 block per wavefront on v_mfma_f32_16x16x4_f32;
-//   4. all waves store L11 and W with coalesced row stores.
+// One workgroup of 4 wavefronts per matrix, the block resident in LDS:
+//   1. all waves stage the (already updated) block with coalesced row loads;
+//      rows past n are the identity;
+//   2. wavefront 0 factors the top-left 64x64 block D1 in registers
+//      (reg_factor: lane = row, pivot column exchanged in the wave);
+//   3. W1 = D1^-1 by 16x16 blocks: the 4 wavefronts invert the four diagonal
+//      16x16 blocks at once (lane = column, forward substitution), then
+//      block rows 1..3 in turn, W_ij = -W_ii sum_{k=j}^{i-1} L_ik W_kj;
+//   4. L21 = A21 W1^T and D2 -= L21 L21^T on v_mfma_f32_16x16x4_f32;
+//   5. wavefront 0 factors D2; W2 = D2^-1 as in 3;
+//   6. W21 = -W2 (L21 W1), so W = L11^-1 is complete;
+//   7. all waves store L11 and W with coalesced row stores.
+// Two 64-pivot chains (~7.5 us each at ~110 ns a pivot, measured with an
+// s_memtime probe) dominate; the MFMA steps add ~1-2 us.
 //
 // Reference counterpart: e62_diag128 (submission.py:4694-4997), which
-// factors a 128x128 block and builds its inverse in one CTA with warp 0
-// forming 32x32 inverses while the other warps update.
+// factors a 128x128 block and builds its inverse in one CTA.
 #include "chol_common.h"
 
 namespace chol {
 namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-constexpr int NB = kNB;
-constexpr int P = NB + 4;
+constexpr int NB = kNB;     // 128
+constexpr int H = NB / 2;   // 64
+constexpr int P = NB + 4;   // LDS pitch of the block buffers
+constexpr int TP = 16 + 4;  // pitch of the per-wave 16x16 staging
 
-// C(16x16) += X Y for 16x16 row-major LDS blocks (pitch P): X rows as float4,
-// Y columns as scalars.  Lane (li, lg) ends with C[4lg+v][li] in acc[v].
-__device__ __forceinline__ floatx4 mma16(const float* X, const float* Y, floatx4 acc, int li, int lg) {
-  const float4 xa = *reinterpret_cast<const float4*>(X + li * P + 4 * lg);
-  const float y0 = Y[(4 * lg + 0) * P + li];
-  const float y1 = Y[(4 * lg + 1) * P + li];
-  const float y2 = Y[(4 * lg + 2) * P + li];
-  const float y3 = Y[(4 * lg + 3) * P + li];
+// C(16x16) += X Y, X rows read as float4 (pitch px), Y columns read as
+// scalars (pitch py), K = 16.  Lane (li, lg) holds C[4lg+v][li] in acc[v].
+__device__ __forceinline__ floatx4 mma_nn(const float* X, int px, const float* Y, int py, floatx4 acc, int li,
+                                          int lg) {
+  const float4 xa = *reinterpret_cast<const float4*>(X + li * px + 4 * lg);
+  const float y0 = Y[(4 * lg + 0) * py + li];
+  const float y1 = Y[(4 * lg + 1) * py + li];
+  const float y2 = Y[(4 * lg + 2) * py + li];
+  const float y3 = Y[(4 * lg + 3) * py + li];
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, y0, acc, 0, 0, 0);
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, y1, acc, 0, 0, 0);
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.z, y2, acc, 0, 0, 0);
@@ -39,11 +45,89 @@ __device__ __forceinline__ floatx4 mma16(const float* X, const float* Y, floatx4
   return acc;
 }
 
+// C(16x16) += X Y^T, both X and Y rows read as float4 (pitch P), K = 16.
+__device__ __forceinline__ floatx4 mma_nt(const float* X, const float* Y, floatx4 acc, int li, int lg) {
+  const float4 xa = *reinterpret_cast<const float4*>(X + li * P + 4 * lg);
+  const float4 yb = *reinterpret_cast<const float4*>(Y + li * P + 4 * lg);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, yb.x, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, yb.y, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.z, yb.z, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.w, yb.w, acc, 0, 0, 0);
+  return acc;
+}
+
+// Wavefront 0 factors the 64x64 block of Ls at (d, d) in registers.
+// Returns the 1-based first failing pivot (0 if none); lanes >= 64 idle.
+__device__ __forceinline__ int factor64(float* Ls, float* cb, int d, int lane) {
+  float a[H];
+#pragma unroll
+  for (int j = 0; j < H; j += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(&Ls[(d + lane) * P + d + j]);
+    a[j] = v.x;
+    a[j + 1] = v.y;
+    a[j + 2] = v.z;
+    a[j + 3] = v.w;
+  }
+  const int bad = reg_factor<H>(a, lane, cb, true);
+#pragma unroll
+  for (int j = 0; j < H; j += 4)
+    *reinterpret_cast<float4*>(&Ls[(d + lane) * P + d + j]) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+  return bad;
+}
+
+// Ws(d..d+64, d..d+64) = inverse of the lower-triangular Ls(d.., d..) block,
+// all 4 wavefronts.  Ends with a workgroup barrier.
+__device__ __forceinline__ void inverse64(const float* Ls, float* Ws, float (*Ts)[16 * TP], int d, int wave,
+                                          int lane, int li, int lg) {
+  {  // four diagonal 16x16 inverses at once, wave w -> block w
+    const int d0 = d + 16 * wave;
+    const int c = li;
+    float w[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const float* Li = &Ls[(d0 + i) * P + d0];
+      float s = (i == c) ? 1.f : 0.f;
+#pragma unroll
+      for (int q = 0; q < i; q += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(Li + q);
+        s = fmaf(-v.x, w[q], s);
+        if (q + 1 < i) s = fmaf(-v.y, w[q + 1], s);
+        if (q + 2 < i) s = fmaf(-v.z, w[q + 2], s);
+        if (q + 3 < i) s = fmaf(-v.w, w[q + 3], s);
+      }
+      w[i] = s * __builtin_amdgcn_rcpf(Li[i]);
+    }
+    if (lane < 16) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) Ws[(d0 + i) * P + d0 + c] = w[i];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 1; i < 4; ++i) {  // block row i, wave j < i
+    if (wave < i) {
+      const int j = wave;
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int k = j; k < i; ++k)
+        acc = mma_nn(&Ls[(d + 16 * i) * P + d + 16 * k], P, &Ws[(d + 16 * k) * P + d + 16 * j], P, acc, li, lg);
+      float* T = Ts[wave];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) T[(4 * lg + v) * TP + li] = acc[v];
+      wave_lds_sync();
+      floatx4 r = {0.f, 0.f, 0.f, 0.f};
+      r = mma_nn(&Ws[(d + 16 * i) * P + d + 16 * i], P, T, TP, r, li, lg);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ws[(d + 16 * i + 4 * lg + v) * P + d + 16 * j + li] = -r[v];
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv) {
   __shared__ __align__(16) float Ls[NB * P];
   __shared__ __align__(16) float Ws[NB * P];
-  __shared__ __align__(16) float Ts[4][16 * P];  // per-wave 16x16 staging
-  __shared__ __align__(16) float cb[NB];
+  __shared__ __align__(16) float Ts[4][16 * TP];  // per-wave 16x16 staging
+  __shared__ __align__(16) float cb[H];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -56,11 +140,11 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   float* Lb = M.L + (int64_t)k0 * M.ld + k0;
   const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0);
 
-  // ---- 1. stage the block: thread covers rows tid/16 + 16u, cols 4*(tid%16) ----
-  const int c4 = (tid & 15) * 4;
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int r = (tid >> 4) + 16 * u;
+  // ---- 1. stage: thread covers rows tid/32 + 8u, columns 4*(tid%32) ----
+  const int c4 = (tid & 31) * 4;
+#pragma unroll 4
+  for (int u = 0; u < NB / 8; ++u) {
+    const int r = (tid >> 5) + 8 * u;
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (r < nb) {
       if (c4 <= r) {
@@ -88,78 +172,97 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   }
   __syncthreads();
 
-  // ---- 2. wavefront 0 factors in registers ----
-  if (wave == 0) {
-    float a[NB];
-#pragma unroll
-    for (int j = 0; j < NB; j += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(&Ls[lane * P + j]);
-      a[j] = v.x;
-      a[j + 1] = v.y;
-      a[j + 2] = v.z;
-      a[j + 3] = v.w;
-    }
-    const int bad = reg_factor<NB>(a, lane, cb, true);
-#pragma unroll
-    for (int j = 0; j < NB; j += 4)
-      *reinterpret_cast<float4*>(&Ls[lane * P + j]) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
-    if (p.info && lane == 0 && bad != 0 && bad <= nb && p.info[M.slot] == 0) p.info[M.slot] = k0 + bad;
-  }
+  // ---- 2-3. D1 and W1 ----
+  int bad = 0;
+  if (wave == 0) bad = factor64(Ls, cb, 0, lane);
   __syncthreads();
+  inverse64(Ls, Ws, Ts, 0, wave, lane, li, lg);
 
-  // ---- 3a. diagonal 16x16 inverses, wave w -> block w, lane c < 16 ----
+  // ---- 4a. L21 = A21 W1^T: 16 output tiles, 4 per wave, K = 64 ----
   {
-    const int d0 = 16 * wave;
-    const int c = li;
-    float w[16];
+    floatx4 acc[4];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const float* Li = &Ls[(d0 + i) * P + d0];
-      float s = (i == c) ? 1.f : 0.f;
+    for (int t = 0; t < 4; ++t) {
+      const int ti = wave, tj = t;  // tile row ti, tile column tj (16x16)
+      acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int q = 0; q < i; q += 4) {
-        const float4 v = *reinterpret_cast<const float4*>(Li + q);
-        s = fmaf(-v.x, w[q], s);
-        if (q + 1 < i) s = fmaf(-v.y, w[q + 1], s);
-        if (q + 2 < i) s = fmaf(-v.z, w[q + 2], s);
-        if (q + 3 < i) s = fmaf(-v.w, w[q + 3], s);
-      }
-      w[i] = s * __builtin_amdgcn_rcpf(Li[i]);
+      for (int kk = 0; kk < H; kk += 16)
+        acc[t] = mma_nt(&Ls[(H + 16 * ti) * P + kk], &Ws[(16 * tj) * P + kk], acc[t], li, lg);
     }
-    if (lane < 16) {
+    __syncthreads();  // every read of A21 done before it is overwritten
 #pragma unroll
-      for (int i = 0; i < 16; ++i) Ws[(d0 + i) * P + d0 + c] = w[i];
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ls[(H + 16 * wave + 4 * lg + v) * P + 16 * t + li] = acc[t][v];
+  }
+  __syncthreads();
+  // ---- 4b. D2 -= L21 L21^T on the 10 lower 16x16 tiles ----
+  for (int t = wave; t < 10; t += 4) {
+    int ti = 0, tj = t;  // t -> (ti >= tj): rows of lengths 1, 2, 3, 4
+    while (tj > ti) {
+      tj -= ti + 1;
+      ++ti;
+    }
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < H; kk += 16)
+      acc = mma_nt(&Ls[(H + 16 * ti) * P + kk], &Ls[(H + 16 * tj) * P + kk], acc, li, lg);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
+      if (ti != tj || j <= i) Ls[(H + i) * P + H + j] -= acc[v];
     }
   }
   __syncthreads();
 
-  // ---- 3b. off-diagonal blocks, block row i = 1..3, wave j < i ----
-#pragma unroll
-  for (int i = 1; i < 4; ++i) {
-    if (wave < i) {
-      const int j = wave;
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int k = j; k < i; ++k)
-        acc = mma16(&Ls[(16 * i) * P + 16 * k], &Ws[(16 * k) * P + 16 * j], acc, li, lg);
-      float* T = Ts[wave];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) T[(4 * lg + v) * P + li] = acc[v];
-      wave_lds_sync();
-      floatx4 r = {0.f, 0.f, 0.f, 0.f};
-      r = mma16(&Ws[(16 * i) * P + 16 * i], T, r, li, lg);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Ws[(16 * i + 4 * lg + v) * P + 16 * j + li] = -r[v];
-    }
-    __syncthreads();
+  // ---- 5. D2 and W2 ----
+  if (wave == 0) {
+    const int bad2 = factor64(Ls, cb, H, lane);
+    if (bad == 0 && bad2 != 0) bad = H + bad2;
   }
+  __syncthreads();
+  inverse64(Ls, Ws, Ts, H, wave, lane, li, lg);
 
-  // ---- 4. store L11 (rows < nb, columns < nb) and W ----
-  float* Wb = winv + b * (NB * NB);
+  // ---- 6. W21 = -W2 (L21 W1); T1 = L21 W1 staged in the unused upper-right
+  //         64x64 block of Ls (zeros above the diagonal, never stored) ----
+  {
+    floatx4 acc[4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int r = (tid >> 4) + 16 * u;
-    const float4 w4 = *reinterpret_cast<const float4*>(&Ws[r * P + c4]);
-    *reinterpret_cast<float4*>(&Wb[r * NB + c4]) = w4;
+    for (int t = 0; t < 4; ++t) {
+      acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < H; kk += 16)
+        acc[t] = mma_nn(&Ls[(H + 16 * wave) * P + kk], P, &Ws[kk * P + 16 * t], P, acc[t], li, lg);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ls[(16 * wave + 4 * lg + v) * P + H + 16 * t + li] = acc[t][v];
+  }
+  __syncthreads();
+  {
+    floatx4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < H; kk += 16)
+        acc[t] = mma_nn(&Ws[(H + 16 * wave) * P + H + kk], P, &Ls[kk * P + H + 16 * t], P, acc[t], li, lg);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ws[(H + 16 * wave + 4 * lg + v) * P + 16 * t + li] = -acc[t][v];
+  }
+  if (p.info && tid == 0 && bad != 0 && bad <= nb && p.info[M.slot] == 0) p.info[M.slot] = k0 + bad;
+  __syncthreads();
+
+  // ---- 7. store L11 (rows < nb, columns < nb, lower chunks) and W ----
+  float* Wb = winv + b * (NB * NB);
+#pragma unroll 4
+  for (int u = 0; u < NB / 8; ++u) {
+    const int r = (tid >> 5) + 8 * u;
+    *reinterpret_cast<float4*>(&Wb[r * NB + c4]) = *reinterpret_cast<const float4*>(&Ws[r * P + c4]);
     if (r < nb && c4 <= r) {
       const float4 v = *reinterpret_cast<const float4*>(&Ls[r * P + c4]);
       float* dst = Lb + (int64_t)r * M.ld + c4;

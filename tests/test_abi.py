@@ -66,7 +66,7 @@ def test_rejects_bad_mode_and_strides():
 def test_rejects_short_workspace():
     lib = _lib.load()
     need = lib.chol_workspace_bytes(4, 300, 0)
-    assert need >= 4 * 64 * 64 * 4  # one 64x64 diagonal-block inverse per matrix
+    assert need >= 4 * 128 * 128 * 4  # one 128x128 diagonal-block inverse per matrix
     assert _potrf(batch=4, n=300, ld=300, bs=300 * 300) == _lib.E_WORKSPACE
     assert _potrf(batch=4, n=300, ld=300, bs=300 * 300, ws=ctypes.c_void_p(0x3000),
                   wsb=need - 1) == _lib.E_WORKSPACE
@@ -79,7 +79,7 @@ def test_rejects_short_workspace():
 def test_tuning_table_validation():
     import numpy as np
     lib = _lib.load()
-    ok = np.array([[65, 256, 1, 10**9, -1], [257, 4096, 1, 8, 128]], dtype=np.int64)
+    ok = np.array([[65, 256, 1, 10**9, -1], [257, 4096, 1, 8, 256]], dtype=np.int64)
     assert lib.chol_set_tuning(ok.ctypes.data, 2) == 0
     bad = np.array([[300, 512, 1, 10, -1]], dtype=np.int64)  # LDS route beyond n = 256
     assert lib.chol_set_tuning(bad.ctypes.data, 1) == _lib.E_ARG
