@@ -45,12 +45,36 @@ struct Mat {
   int64_t slot;  // index into info
 };
 
+// Marks a pointer as global memory.  A pointer loaded from the ragged
+// descriptor table is otherwise generic, and every access through it
+// becomes a flat_load/flat_store, which the hardware counts against both
+// the vector-memory and the LDS wait counters: an LDS wait then also waits
+// for in-flight global prefetches.  The round trip through address space 1
+// lets the compiler emit global_load/global_store.
+template <typename T>
+__device__ __forceinline__ T* as_global(T* q) {
+  return (T*)((__attribute__((address_space(1))) T*)q);
+}
+
+// Global loads/stores through address-space-1 pointers: always
+// global_load/global_store, whatever the compiler could infer about `q`.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) fvec4 gfvec4;
+__device__ __forceinline__ float gld(const float* q) { return *(const gfloat*)q; }
+__device__ __forceinline__ float4 gld4(const float* q) {
+  const fvec4 v = *(const gfvec4*)q;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst(float* q, float v) { *(gfloat*)q = v; }
+__device__ __forceinline__ void gst4(float* q, float4 v) { *(gfvec4*)q = fvec4{v.x, v.y, v.z, v.w}; }
+
 __device__ __forceinline__ Mat mat_at(const Batch& p, int64_t b) {
   if (p.Ap) {
     const int n = p.ns[b];
-    return Mat{p.Ap[b], p.Lp[b], n, (int64_t)n, p.orig ? (int64_t)p.orig[b] : b};
+    return Mat{as_global(p.Ap[b]), as_global(p.Lp[b]), n, (int64_t)n, p.orig ? (int64_t)p.orig[b] : b};
   }
-  return Mat{p.A + b * p.bs, p.L + b * p.bs, p.n, p.ld, b};
+  return Mat{as_global(p.A + b * p.bs), as_global(p.L + b * p.bs), p.n, p.ld, b};
 }
 
 // Compiler-level barrier between LDS traffic of lanes of one wavefront.

@@ -140,9 +140,31 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   float* Lb = M.L + (int64_t)k0 * M.ld + k0;
   const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0);
 
-  // ---- 1. stage: thread covers rows tid/32 + 8u, columns 4*(tid%32) ----
+  // ---- 1. stage: thread covers rows tid/32 + 8u, columns 4*(tid%32).
+  //         All 16 loads are issued before the first LDS store, so the
+  //         block costs one memory latency, not one per few rows. ----
   const int c4 = (tid & 31) * 4;
-#pragma unroll 4
+  float4 stg[NB / 8];
+  if (vec && nb == NB) {
+    // Full aligned block: every load unconditional (chunks above the
+    // diagonal read column 0 of their row and are zeroed afterwards).
+#pragma unroll
+    for (int u = 0; u < NB / 8; ++u) {
+      const int r = (tid >> 5) + 8 * u;
+      stg[u] = gld4(Lb + (int64_t)r * M.ld + (c4 <= r ? c4 : 0));
+    }
+#pragma unroll
+    for (int u = 0; u < NB / 8; ++u) {
+      const int r = (tid >> 5) + 8 * u;
+      float4 v = stg[u];
+      if (c4 > r) v.x = 0.f;
+      if (c4 + 1 > r) v.y = 0.f;
+      if (c4 + 2 > r) v.z = 0.f;
+      if (c4 + 3 > r) v.w = 0.f;
+      stg[u] = v;
+    }
+  } else {
+#pragma unroll
   for (int u = 0; u < NB / 8; ++u) {
     const int r = (tid >> 5) + 8 * u;
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -150,12 +172,12 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
       if (c4 <= r) {
         const float* src = Lb + (int64_t)r * M.ld + c4;
         if (vec && c4 + 3 < nb) {
-          v = *reinterpret_cast<const float4*>(src);
+          v = gld4(src);
         } else {
-          v.x = src[0];
-          if (c4 + 1 < nb) v.y = src[1];
-          if (c4 + 2 < nb) v.z = src[2];
-          if (c4 + 3 < nb) v.w = src[3];
+          v.x = gld(src);
+          if (c4 + 1 < nb) v.y = gld(src + 1);
+          if (c4 + 2 < nb) v.z = gld(src + 2);
+          if (c4 + 3 < nb) v.w = gld(src + 3);
         }
         if (c4 + 1 > r) v.y = 0.f;
         if (c4 + 2 > r) v.z = 0.f;
@@ -167,7 +189,13 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
       v.z = (c4 + 2 == r) ? 1.f : 0.f;
       v.w = (c4 + 3 == r) ? 1.f : 0.f;
     }
-    *reinterpret_cast<float4*>(&Ls[r * P + c4]) = v;
+    stg[u] = v;
+  }
+  }
+#pragma unroll
+  for (int u = 0; u < NB / 8; ++u) {
+    const int r = (tid >> 5) + 8 * u;
+    *reinterpret_cast<float4*>(&Ls[r * P + c4]) = stg[u];
     *reinterpret_cast<float4*>(&Ws[r * P + c4]) = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __syncthreads();
@@ -262,17 +290,17 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 #pragma unroll 4
   for (int u = 0; u < NB / 8; ++u) {
     const int r = (tid >> 5) + 8 * u;
-    *reinterpret_cast<float4*>(&Wb[r * NB + c4]) = *reinterpret_cast<const float4*>(&Ws[r * P + c4]);
+    gst4(&Wb[r * NB + c4], *reinterpret_cast<const float4*>(&Ws[r * P + c4]));
     if (r < nb && c4 <= r) {
       const float4 v = *reinterpret_cast<const float4*>(&Ls[r * P + c4]);
       float* dst = Lb + (int64_t)r * M.ld + c4;
       if (vec && c4 + 3 < nb) {
-        *reinterpret_cast<float4*>(dst) = v;
+        gst4(dst, v);
       } else {
-        dst[0] = v.x;
-        if (c4 + 1 < nb) dst[1] = v.y;
-        if (c4 + 2 < nb) dst[2] = v.z;
-        if (c4 + 3 < nb) dst[3] = v.w;
+        gst(dst, v.x);
+        if (c4 + 1 < nb) gst(dst + 1, v.y);
+        if (c4 + 2 < nb) gst(dst + 2, v.z);
+        if (c4 + 3 < nb) gst(dst + 3, v.w);
       }
     }
   }

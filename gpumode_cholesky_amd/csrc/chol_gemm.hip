@@ -45,12 +45,12 @@ __device__ __forceinline__ void load_rows(float4 (&reg)[ROWS * KC / 4 / kThreads
     if (gr < rlim) {
       const float* src = base + (int64_t)gr * ld + k + q;
       if constexpr (VEC) {
-        v = *reinterpret_cast<const float4*>(src);
+        v = gld4(src);
       } else {
-        v.x = src[0];
-        v.y = src[1];
-        v.z = src[2];
-        v.w = src[3];
+        v.x = gld(src);
+        v.y = gld(src + 1);
+        v.z = gld(src + 2);
+        v.w = gld(src + 3);
       }
     }
     reg[u] = v;
@@ -101,14 +101,14 @@ __device__ __forceinline__ void epilogue(const floatx4 (&acc)[FM][FN], float* C,
 #pragma unroll
       for (int b = 0; b < FN; ++b)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) old[a][b][v] = C[off[a][b][v]];
+        for (int v = 0; v < 4; ++v) old[a][b][v] = gld(C + off[a][b][v]);
 #pragma unroll
     for (int a = 0; a < FM; ++a)
 #pragma unroll
       for (int b = 0; b < FN; ++b)
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-          if (ok[a][b][v]) C[off[a][b][v]] = old[a][b][v] - acc[a][b][v];
+          if (ok[a][b][v]) gst(C + off[a][b][v], old[a][b][v] - acc[a][b][v]);
   } else {
 #pragma unroll
     for (int a = 0; a < FM; ++a)
@@ -116,7 +116,7 @@ __device__ __forceinline__ void epilogue(const floatx4 (&acc)[FM][FN], float* C,
       for (int b = 0; b < FN; ++b)
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-          if (ok[a][b][v]) C[off[a][b][v]] = acc[a][b][v];
+          if (ok[a][b][v]) gst(C + off[a][b][v], acc[a][b][v]);
   }
 }
 

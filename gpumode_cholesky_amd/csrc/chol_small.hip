@@ -35,7 +35,7 @@ __device__ __forceinline__ void slot_rows(const Batch& p, int64_t b, bool live, 
   for (int j = 0; j < NP; ++j) {
     float v;
     if (r < n)
-      v = (live && j <= r) ? src[j] : 0.f;
+      v = (live && j <= r) ? gld(src + j) : 0.f;
     else
       v = (j == r) ? 1.f : 0.f;
     a[j] = v;
@@ -50,7 +50,7 @@ __device__ __forceinline__ void slot_store(const Batch& p, int64_t b, bool live,
   float* dst = M.L + (int64_t)r * M.ld;
 #pragma unroll
   for (int j = 0; j < NP; ++j)
-    if (j < M.n) dst[j] = a[j];
+    if (j < M.n) gst(dst + j, a[j]);
 }
 
 template <int NP, bool VEC>
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
       const int col = e % NP;
       const int rr = row % NP;
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (b0 + row / NP < p.batch && col <= rr) v = *reinterpret_cast<const float4*>(src + e);
+      if (b0 + row / NP < p.batch && col <= rr) v = gld4(src + e);
       *reinterpret_cast<float4*>(st + row * PITCH + col) = v;
     }
     wave_lds_sync();
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
       const int row = e / NP;
       const int col = e % NP;
       if (b0 + row / NP < p.batch)
-        *reinterpret_cast<float4*>(dst + e) = *reinterpret_cast<const float4*>(st + row * PITCH + col);
+        gst4(dst + e, *reinterpret_cast<const float4*>(st + row * PITCH + col));
     }
   } else {
     slot_store<NP>(p, b, live, r, a);
