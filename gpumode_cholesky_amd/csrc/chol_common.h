@@ -87,48 +87,89 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Right-looking factorisation of an NP x NP block held one row per lane:
-// a[] is the lane's row r of the block (lower part; the upper part is never
-// read by other lanes), cb the per-block LDS column buffer (NP floats,
-// 16-byte aligned), `writer` whether this lane owns row r (duplicate lanes
-// compute but do not write).  Pivot step k: every row writes its column-k
-// entry, reads back the pivot c[k] and c[k+1..NP) with broadcast reads,
-// and folds a[j] -= (a[k]/c[k]) c[j].  The pivot reciprocal square root is
-// v_rsq_f32 (~1 ulp); L[r][k] = a[k] * rsq(c[k]) for r > k, sqrt(c[k]) from
-// c[k] * rsq(c[k]) plus one Newton correction for r == k (exact for perfect
-// squares), 0 above the diagonal.  A non-positive pivot
-// yields NaN from there on; the return value is the 1-based first failing
-// pivot (0 if none).
+// Right-looking factorisation of the leading NP columns of a block held one
+// row per lane: a[] holds columns 0..NP-1 of the lane's row r (lower part;
+// the upper part is never read by other lanes), cb the per-block LDS column
+// buffer (one float per row, 16-byte aligned), `writer` whether this lane
+// owns row r (duplicate lanes compute but do not write).  Rows r >= NP
+// (a wave of 64 rows over NP = 32 columns) come out as the panel below the
+// factor, L[r][k] for k < NP.  Pivot step k: every row publishes its
+// column-k entry, reads back the pivot c[k] and c[k+1..NP) with broadcast
+// reads, and folds a[j] -= (a[k]/c[k]) c[j].
+//
+// Software pipelined: as soon as step k's multiplier is known, column k+1
+// is brought up to date, published and its read-back issued; the bulk of
+// step k's updates then runs while that LDS round trip is in flight.  The
+// scheduling barriers keep the compiler from hoisting the next step's
+// pivot arithmetic (and with it the LDS wait) above those updates.  The
+// pivot reciprocal square root is v_rsq_f32 (~1 ulp); L[r][k] =
+// a[k] * rsq(c[k]) for r > k, sqrt(c[k]) from c[k] * rsq(c[k]) plus one
+// Newton correction for r == k (exact for perfect squares), 0 above the
+// diagonal.  A non-positive pivot yields NaN from there on; the return
+// value is the 1-based first failing pivot (0 if none).
 template <int NP>
 __device__ __forceinline__ int reg_factor(float (&a)[NP], int r, float* cb, bool writer) {
   int bad = 0;
+  float c[NP];
+  if (writer) cb[r] = a[0];
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < NP; q += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(cb + q);
+    c[q] = v.x;
+    c[q + 1] = v.y;
+    c[q + 2] = v.z;
+    c[q + 3] = v.w;
+  }
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
-    if (writer) cb[r] = a[k];
-    wave_lds_sync();
-    float c[NP];
-#pragma unroll
-    for (int q = (k / 4) * 4; q < NP; q += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(cb + q);
-      c[q] = v.x;
-      c[q + 1] = v.y;
-      c[q + 2] = v.z;
-      c[q + 3] = v.w;
-    }
     const float piv = c[k];
     if (bad == 0 && !(piv > 0.f)) bad = k + 1;
     const float is = __builtin_amdgcn_rsqf(piv);
     const float lk = a[k] * is;
     const float t = (r > k) ? lk * is : 0.f;
+    float cn[NP];
+    if (k + 1 < NP) {
+      a[k + 1] = fmaf(-t, c[k + 1], a[k + 1]);
+      wave_lds_sync();  // this step's reads of cb were issued before the write
+      if (writer) cb[r] = a[k + 1];
+      wave_lds_sync();
 #pragma unroll
-    for (int j = k + 1; j < NP; ++j) a[j] = fmaf(-t, c[j], a[j]);
+      for (int q = ((k + 1) / 4) * 4; q < NP; q += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(cb + q);
+        cn[q] = v.x;
+        cn[q + 1] = v.y;
+        cn[q + 2] = v.z;
+        cn[q + 3] = v.w;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = k + 2; j < NP; ++j) a[j] = fmaf(-t, c[j], a[j]);
     // diagonal: piv * rsq(piv), one Newton step on the residual piv - s^2
     const float s0 = piv * is;
     const float s = fmaf(0.5f * is, fmaf(-s0, s0, piv), s0);
     a[k] = (r > k) ? lk : ((r == k) ? s : 0.f);
-    wave_lds_sync();
+    __builtin_amdgcn_sched_barrier(0);
+    if (k + 1 < NP) {
+#pragma unroll
+      for (int q = ((k + 1) / 4) * 4; q < NP; ++q) c[q] = cn[q];
+    }
   }
   return bad;
+}
+
+// Tiles of the lower region {(i, j): col0 <= j < col1, j <= i < n} in T x T
+// tiles (ti >= tj), enumerated column by column: column tj holds TI - tj.
+__device__ __forceinline__ void decode_tile(int idx, int TI, int& ti, int& tj) {
+  const float twoTI1 = 2.f * TI + 1.f;
+  const float disc = fmaxf(twoTI1 * twoTI1 - 8.f * (float)idx, 0.f);
+  int t = (int)floorf((twoTI1 - sqrtf(disc)) * 0.5f);
+  if (t < 0) t = 0;
+  while (t > 0 && (int64_t)t * TI - (int64_t)t * (t - 1) / 2 > idx) --t;
+  while ((int64_t)(t + 1) * TI - (int64_t)(t + 1) * t / 2 <= idx) ++t;
+  tj = t;
+  ti = t + (idx - (int)((int64_t)t * TI - (int64_t)t * (t - 1) / 2));
 }
 
 inline int launch_status(const char* what) {
@@ -153,6 +194,10 @@ int launch_panel(const Batch& p, int k0, const float* winv,
 int launch_syrk(const Batch& p, int col0, int col1, int k0, int kl, bool split,
                 hipStream_t s);
 int launch_zero_info(int32_t* info, int64_t batch, hipStream_t s);
+// 256x256-tile form of the SYRK for large updates (K a multiple of 32);
+// syrk256_tiles: its workgroup count (tiles x batch).
+int64_t syrk256_tiles(const Batch& p, int col0, int col1);
+int launch_syrk256(const Batch& p, int col0, int col1, int k0, int kl, bool split, bool vec, hipStream_t s);
 
 // Contiguous batches (ld = n).  tri_inverse: X = L^-1, ws = batch*n*n floats.
 // backward: Abar from L, Lbar; ws = 2 (Linv given) or 3 batch*n*n floats.

@@ -56,22 +56,79 @@ __device__ __forceinline__ floatx4 mma_nt(const float* X, const float* Y, floatx
   return acc;
 }
 
-// Wavefront 0 factors the 64x64 block of Ls at (d, d) in registers.
-// Returns the 1-based first failing pivot (0 if none); lanes >= 64 idle.
+// Wavefront 0 factors the 64x64 block of Ls at (d, d) as two 32-pivot
+// register chains around one MFMA update:
+//   1. lane = row of the 64x32 left panel: reg_factor<32> yields L11 and,
+//      for rows 32..63, L21 (the panel's own triangular solve);
+//   2. A22 -= L21 L21^T (32x32, K = 32) on v_mfma_f32_16x16x4_f32;
+//   3. lanes 0..31 = rows of A22: reg_factor<32> yields L22.
+// A pivot step's cost is one LDS round trip plus the row updates, about
+// half of which vanish with 32-wide rows; one 64-wide chain costs about
+// twice as much.  Returns the 1-based first failing pivot (0 if none).
 __device__ __forceinline__ int factor64(float* Ls, float* cb, int d, int lane) {
-  float a[H];
+  constexpr int Q = H / 2;  // 32
+  const int li = lane & 15, lg = lane >> 4;
+  int bad;
+  {
+    float a[Q];
+    float* row = &Ls[(d + lane) * P + d];
 #pragma unroll
-  for (int j = 0; j < H; j += 4) {
-    const float4 v = *reinterpret_cast<const float4*>(&Ls[(d + lane) * P + d + j]);
-    a[j] = v.x;
-    a[j + 1] = v.y;
-    a[j + 2] = v.z;
-    a[j + 3] = v.w;
+    for (int j = 0; j < Q; j += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(row + j);
+      a[j] = v.x;
+      a[j + 1] = v.y;
+      a[j + 2] = v.z;
+      a[j + 3] = v.w;
+    }
+    bad = reg_factor<Q>(a, lane, cb, true);
+#pragma unroll
+    for (int j = 0; j < Q; j += 4) *reinterpret_cast<float4*>(row + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
   }
-  const int bad = reg_factor<H>(a, lane, cb, true);
+  wave_lds_sync();
+  {
+    // lower 16x16 tiles (0,0), (1,0), (1,1) of A22; the update writes only
+    // columns d+32.., the products read only columns d..d+31
+    floatx4 acc[3];
 #pragma unroll
-  for (int j = 0; j < H; j += 4)
-    *reinterpret_cast<float4*>(&Ls[(d + lane) * P + d + j]) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+    for (int t = 0; t < 3; ++t) {
+      const int ti = (t + 1) / 2, tj = t / 2;
+      acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < Q; kk += 16)
+        acc[t] = mma_nt(&Ls[(d + Q + 16 * ti) * P + d + kk], &Ls[(d + Q + 16 * tj) * P + d + kk], acc[t], li, lg);
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const int ti = (t + 1) / 2, tj = t / 2;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
+        if (ti != tj || j <= i) Ls[(d + Q + i) * P + d + Q + j] -= acc[t][v];
+      }
+    }
+  }
+  wave_lds_sync();
+  {
+    // lanes 32..63 mirror rows 0..31 of A22 and do not write
+    const int r = lane & (Q - 1);
+    float a[Q];
+    float* row = &Ls[(d + Q + r) * P + d + Q];
+#pragma unroll
+    for (int j = 0; j < Q; j += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(row + j);
+      a[j] = v.x;
+      a[j + 1] = v.y;
+      a[j + 2] = v.z;
+      a[j + 3] = v.w;
+    }
+    const int bad2 = reg_factor<Q>(a, r, cb, lane < Q);
+    if (lane < Q) {
+#pragma unroll
+      for (int j = 0; j < Q; j += 4)
+        *reinterpret_cast<float4*>(row + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+    }
+    if (bad == 0 && bad2 != 0) bad = Q + bad2;
+  }
   return bad;
 }
 
@@ -123,11 +180,23 @@ __device__ __forceinline__ void inverse64(const float* Ls, float* Ws, float (*Ts
   }
 }
 
+// LDS of the kernel, in this order on purpose: the pivot-column buffer and
+// the staging tiles sit below 64 KiB, where every access of the register
+// factorisation encodes its address as an immediate offset (no per-read
+// address register).
+struct __align__(16) DiagLds {
+  float cb[H];
+  float Ts[4][16 * TP];  // per-wave 16x16 staging
+  float Ls[NB * P];
+  float Ws[NB * P];
+};
+
 __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv) {
-  __shared__ __align__(16) float Ls[NB * P];
-  __shared__ __align__(16) float Ws[NB * P];
-  __shared__ __align__(16) float Ts[4][16 * TP];  // per-wave 16x16 staging
-  __shared__ __align__(16) float cb[H];
+  __shared__ DiagLds sm;
+  float* const cb = sm.cb;
+  float(*const Ts)[16 * TP] = sm.Ts;
+  float* const Ls = sm.Ls;
+  float* const Ws = sm.Ws;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;

@@ -30,16 +30,16 @@ constexpr int KC = 32;        // K chunk staged in LDS
 constexpr int KP = KC + 4;    // LDS row pitch (floats)
 constexpr int kThreads = 256;
 
-template <int ROWS, bool VEC>
-__device__ __forceinline__ void load_rows(float4 (&reg)[ROWS * KC / 4 / kThreads], const float* base,
+template <int ROWS, bool VEC, int KW = KC>
+__device__ __forceinline__ void load_rows(float4 (&reg)[ROWS * KW / 4 / kThreads], const float* base,
                                           int64_t ld, int row0, int rlim, int k, int tid) {
-  // ROWS x KC floats as float4 chunks; thread t takes chunks t, t+256, ...
-  constexpr int PER = ROWS * KC / 4 / kThreads;
+  // ROWS x KW floats as float4 chunks; thread t takes chunks t, t+256, ...
+  constexpr int PER = ROWS * KW / 4 / kThreads;
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int c = tid + u * kThreads;
-    const int r = c / (KC / 4);
-    const int q = (c % (KC / 4)) * 4;
+    const int r = c / (KW / 4);
+    const int q = (c % (KW / 4)) * 4;
     const int gr = row0 + r;
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (gr < rlim) {
@@ -57,16 +57,16 @@ __device__ __forceinline__ void load_rows(float4 (&reg)[ROWS * KC / 4 / kThreads
   }
 }
 
-template <int ROWS>
-__device__ __forceinline__ void store_rows(float* __restrict__ lds, const float4 (&reg)[ROWS * KC / 4 / kThreads],
+template <int ROWS, int KW = KC, int PITCH = KP>
+__device__ __forceinline__ void store_rows(float* __restrict__ lds, const float4 (&reg)[ROWS * KW / 4 / kThreads],
                                            int tid) {
-  constexpr int PER = ROWS * KC / 4 / kThreads;
+  constexpr int PER = ROWS * KW / 4 / kThreads;
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int c = tid + u * kThreads;
-    const int r = c / (KC / 4);
-    const int q = (c % (KC / 4)) * 4;
-    *reinterpret_cast<float4*>(lds + r * KP + q) = reg[u];
+    const int r = c / (KW / 4);
+    const int q = (c % (KW / 4)) * 4;
+    *reinterpret_cast<float4*>(lds + r * PITCH + q) = reg[u];
   }
 }
 
@@ -215,21 +215,21 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 constexpr int KPH = KC + 8;  // bf16 LDS row pitch (80 bytes)
 
-template <int ROWS>
+template <int ROWS, int KW = KC, int PITCH = KPH>
 __device__ __forceinline__ void store_rows_split(__bf16* __restrict__ hi, __bf16* __restrict__ lo,
-                                                 const float4 (&reg)[ROWS * KC / 4 / kThreads], int tid) {
-  constexpr int PER = ROWS * KC / 4 / kThreads;
+                                                 const float4 (&reg)[ROWS * KW / 4 / kThreads], int tid) {
+  constexpr int PER = ROWS * KW / 4 / kThreads;
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int c = tid + u * kThreads;
-    const int r = c / (KC / 4);
-    const int q = (c % (KC / 4)) * 4;
+    const int r = c / (KW / 4);
+    const int q = (c % (KW / 4)) * 4;
     const float4 v = reg[u];
     const __bf16 h0 = (__bf16)v.x, h1 = (__bf16)v.y, h2 = (__bf16)v.z, h3 = (__bf16)v.w;
     const __bf16 l0 = (__bf16)(v.x - (float)h0), l1 = (__bf16)(v.y - (float)h1);
     const __bf16 l2 = (__bf16)(v.z - (float)h2), l3 = (__bf16)(v.w - (float)h3);
-    bf16x2* ph = reinterpret_cast<bf16x2*>(hi + r * KPH + q);
-    bf16x2* pl = reinterpret_cast<bf16x2*>(lo + r * KPH + q);
+    bf16x2* ph = reinterpret_cast<bf16x2*>(hi + r * PITCH + q);
+    bf16x2* pl = reinterpret_cast<bf16x2*>(lo + r * PITCH + q);
     ph[0] = bf16x2{h0, h1};
     ph[1] = bf16x2{h2, h3};
     pl[0] = bf16x2{l0, l1};
@@ -312,17 +312,156 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda
   epilogue<FM, FN, 0>(acc, Cbase, ldc, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
 }
 
-// Tiles of the lower region {(i, j): col0 <= j < col1, j <= i < n} in 128x128
-// tiles (ti >= tj), enumerated column by column: column tj holds TI - tj.
-__device__ __forceinline__ void decode_tile(int idx, int TI, int& ti, int& tj) {
-  const float twoTI1 = 2.f * TI + 1.f;
-  const float disc = fmaxf(twoTI1 * twoTI1 - 8.f * (float)idx, 0.f);
-  int t = (int)floorf((twoTI1 - sqrtf(disc)) * 0.5f);
-  if (t < 0) t = 0;
-  while (t > 0 && (int64_t)t * TI - (int64_t)t * (t - 1) / 2 > idx) --t;
-  while ((int64_t)(t + 1) * TI - (int64_t)(t + 1) * t / 2 <= idx) ++t;
-  tj = t;
-  ti = t + (idx - (int)((int64_t)t * TI - (int64_t)t * (t - 1) / 2));
+// ------------------------------------------------------ one-shot K = 128 ----
+// Latency form of the per-step launches (K = 128, too few tiles to fill the
+// GPU, as for one or a few large matrices): a workgroup requests all of its
+// A and B rows (64 KiB for a 64x64 tile) and its C tile at once, stages A
+// and B into LDS in one pass and runs the whole K extent without further
+// barriers.  A chunked pipeline pays one global-memory round trip per
+// 32-wide chunk plus one for C, and with a few short tiles per CU those
+// round trips, not the matrix cores, set the time.
+constexpr int K1 = kNB;       // K extent of the one-shot form
+constexpr int K1P = K1 + 4;   // FP32 LDS row pitch (floats)
+constexpr int K1H = K1 + 8;   // BF16 LDS row pitch (elements)
+
+// MODE 0: C -= A B^T on the lower part (i >= j), i < ilim, j < jlim.
+// MODE 1: C  = A B^T, i < ilim (panel; A and C alias row-for-row).
+// SPLIT: BF16x3 products as in gemm_tile_bf16x3 (MODE 0 only).
+template <int BM, int BN, int MODE, bool VEC, bool SPLIT>
+__device__ __forceinline__ void gemm_tile_once(const float* Abase, int64_t lda, int a_row0, int a_lim, int ka,
+                                               const float* Bbase, int64_t ldb, int b_row0, int b_lim, int kb,
+                                               float* Cbase, int64_t ldc, int c_row0, int c_col0, int c_ilim,
+                                               int c_jlim) {
+  static_assert(!SPLIT || MODE == 0, "split products are used for the trailing update only");
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int PA = BM * K1 / 4 / kThreads;
+  constexpr int PB = BN * K1 / 4 / kThreads;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * WM;
+  const int wn = (wave & 1) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+
+  float4 ra[PA], rb[PB];
+  load_rows<BM, VEC, K1>(ra, Abase, lda, a_row0, a_lim, ka, tid);
+  load_rows<BN, VEC, K1>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
+
+  // C tile of this wave: offsets, masks and (MODE 0) the old values, whose
+  // loads fly together with the operand loads.  A masked element reads
+  // C[0][0], always valid; only the stores are masked.
+  const int r0 = c_row0 + wm, cq = c_col0 + wn;
+  int64_t off[FM][FN][4];
+  bool ok[FM][FN][4];
+  float old[FM][FN][4];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = r0 + a * 16 + 4 * lg + v;
+        const int j = cq + b * 16 + li;
+        const bool m = (MODE == 0) ? (i < c_ilim && j < c_jlim && j <= i) : (i < c_ilim);
+        ok[a][b][v] = m;
+        off[a][b][v] = m ? (int64_t)i * ldc + j : 0;
+        if constexpr (MODE == 0) old[a][b][v] = gld(Cbase + off[a][b][v]);
+      }
+
+  floatx4 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  if constexpr (SPLIT) {
+    __shared__ __align__(16) __bf16 Ah[BM * K1H];
+    __shared__ __align__(16) __bf16 Al[BM * K1H];
+    __shared__ __align__(16) __bf16 Bh[BN * K1H];
+    __shared__ __align__(16) __bf16 Bl[BN * K1H];
+    store_rows_split<BM, K1, K1H>(Ah, Al, ra, tid);
+    store_rows_split<BN, K1, K1H>(Bh, Bl, rb, tid);
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < K1; ks += 32) {
+      bf16x8 fah[FM], fal[FM], fbh[FN], fbl[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) {
+        const int o = (wm + a * 16 + li) * K1H + ks + 8 * lg;
+        fah[a] = *reinterpret_cast<const bf16x8*>(&Ah[o]);
+        fal[a] = *reinterpret_cast<const bf16x8*>(&Al[o]);
+      }
+#pragma unroll
+      for (int b = 0; b < FN; ++b) {
+        const int o = (wn + b * 16 + li) * K1H + ks + 8 * lg;
+        fbh[b] = *reinterpret_cast<const bf16x8*>(&Bh[o]);
+        fbl[b] = *reinterpret_cast<const bf16x8*>(&Bl[o]);
+      }
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fal[a], fbh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbh[b], acc[a][b], 0, 0, 0);
+        }
+    }
+  } else {
+    __shared__ __align__(16) float As[BM * K1P];
+    __shared__ __align__(16) float Bs[BN * K1P];
+    store_rows<BM, K1, K1P>(As, ra, tid);
+    store_rows<BN, K1, K1P>(Bs, rb, tid);
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < K1; ks += 16) {
+      float4 fa[FM], fb[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+        fa[a] = *reinterpret_cast<const float4*>(As + (wm + a * 16 + li) * K1P + ks + 4 * lg);
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+        fb[b] = *reinterpret_cast<const float4*>(Bs + (wn + b * 16 + li) * K1P + ks + 4 * lg);
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].x, fb[b].x, acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].y, fb[b].y, acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].z, fb[b].z, acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].w, fb[b].w, acc[a][b], 0, 0, 0);
+    }
+  }
+
+  // The panel (MODE 1) overwrites its own A rows: every wave staged them
+  // into LDS before the barrier above, so these stores cannot race.
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (ok[a][b][v]) gst(Cbase + off[a][b][v], old[a][b][v] - acc[a][b][v]);
+  } else {
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (ok[a][b][v]) gst(Cbase + off[a][b][v], acc[a][b][v]);
+  }
 }
 
 // SYRK tile T x T (128, or 64 when a launch has too few 128-tiles to fill
@@ -358,6 +497,36 @@ __global__ __launch_bounds__(kThreads) void k_panel_mfma(Batch p, int k0, const 
   gemm_tile<PT, kNB, 1, VEC>(Lb, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, M.ld, i0, k0, M.n, k0 + kNB);
 }
 
+// One-shot forms (K = 128) of the two kernels above.
+template <int T, bool VEC, bool SPLIT>
+__global__ __launch_bounds__(kThreads) void k_syrk_once(Batch p, int col0, int col1, int k0) {
+  const int64_t b = blockIdx.y;
+  const Mat M = mat_at(p, b);
+  const int c1 = col1 < 0 ? M.n : col1;
+  if (c1 <= col0) return;
+  const int TI = (M.n - col0 + T - 1) / T;
+  const int TJ = (c1 - col0 + T - 1) / T;
+  if ((int64_t)blockIdx.x >= (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) return;
+  int ti, tj;
+  decode_tile(blockIdx.x, TI, ti, tj);
+  float* Lb = M.L;
+  const int i0 = col0 + T * ti;
+  const int j0 = col0 + T * tj;
+  gemm_tile_once<T, T, 0, VEC, SPLIT>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, Lb, M.ld, i0, j0, M.n, c1);
+}
+
+template <int PT, bool VEC>
+__global__ __launch_bounds__(kThreads) void k_panel_once(Batch p, int k0, const float* __restrict__ winv) {
+  const int64_t b = blockIdx.y;
+  const Mat M = mat_at(p, b);
+  const int i0 = k0 + kNB + PT * blockIdx.x;
+  if (i0 >= M.n) return;
+  float* Lb = M.L;
+  const float* Wb = winv + b * (kNB * kNB);
+  gemm_tile_once<PT, kNB, 1, VEC, false>(Lb, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, Lb, M.ld, i0, k0, M.n,
+                                         k0 + kNB);
+}
+
 bool vec_ok(const Batch& p) {
   if (p.Ap) return p.ld % 4 == 0;  // ragged: the host sets ld = 4 when every matrix is aligned
   return (reinterpret_cast<uintptr_t>(p.L) & 15) == 0 && (p.ld % 4) == 0 && (p.bs % 4) == 0;
@@ -387,11 +556,12 @@ int launch_panel(const Batch& p, int k0, const float* winv, hipStream_t s) {
   const bool small = (int64_t)((rows + 127) / 128) * p.batch < kFillWGs;
   const bool vec = vec_ok(p);
   if (small) {
-    dim3 grid((unsigned)((rows + 63) / 64), (unsigned)p.batch);
+    // one-shot 32-row tiles: W (64 KiB) and the rows staged in one pass
+    dim3 grid((unsigned)((rows + 31) / 32), (unsigned)p.batch);
     if (vec)
-      hipLaunchKernelGGL((k_panel_mfma<64, true>), grid, dim3(kThreads), 0, s, p, k0, winv);
+      hipLaunchKernelGGL((k_panel_once<32, true>), grid, dim3(kThreads), 0, s, p, k0, winv);
     else
-      hipLaunchKernelGGL((k_panel_mfma<64, false>), grid, dim3(kThreads), 0, s, p, k0, winv);
+      hipLaunchKernelGGL((k_panel_once<32, false>), grid, dim3(kThreads), 0, s, p, k0, winv);
   } else {
     dim3 grid((unsigned)((rows + 127) / 128), (unsigned)p.batch);
     if (vec)
@@ -413,6 +583,26 @@ int launch_syrk(const Batch& p, int col0, int col1, int k0, int kl, bool split, 
   const int64_t tiles128 = ((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) * p.batch;
   const bool small = tiles128 < kFillWGs;
   const bool vec = vec_ok(p);
+  if (!small && kl >= 256 && syrk256_tiles(p, col0, col1) >= 256)
+    return launch_syrk256(p, col0, col1, k0, kl, split, vec, s);
+  if (small && kl == K1) {
+    // one-shot 64x64 tiles (the per-step update of one or a few matrices)
+    const int TI = (p.n - col0 + 63) / 64;
+    const int TJ = ((col1 < 0 ? p.n : col1) - col0 + 63) / 64;
+    dim3 grid((unsigned)((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2), (unsigned)p.batch);
+    if (split) {
+      if (vec)
+        hipLaunchKernelGGL((k_syrk_once<64, true, true>), grid, dim3(kThreads), 0, s, p, col0, col1, k0);
+      else
+        hipLaunchKernelGGL((k_syrk_once<64, false, true>), grid, dim3(kThreads), 0, s, p, col0, col1, k0);
+    } else {
+      if (vec)
+        hipLaunchKernelGGL((k_syrk_once<64, true, false>), grid, dim3(kThreads), 0, s, p, col0, col1, k0);
+      else
+        hipLaunchKernelGGL((k_syrk_once<64, false, false>), grid, dim3(kThreads), 0, s, p, col0, col1, k0);
+    }
+    return launch_status("k_syrk_once");
+  }
   if (split) {
     if (small)
       syrk_launch<64, true>(p, col0, col1, k0, kl, vec, s);

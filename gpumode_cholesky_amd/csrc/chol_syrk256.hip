@@ -1,0 +1,283 @@
+// chol_syrk256.hip — the large trailing update C -= P P^T in 256x256 tiles.
+//
+// The outer-block updates of big matrices (K = the outer block width, 256
+// and more, thousands of tiles) are bound by how fast the CUs can pull
+// operands, not by the matrix cores: a 128x128 tile does 32 useful flops per
+// byte of FP32 operand it loads, which at the BF16x3 rate needs ~42 B per
+// clock per CU.  A 256x256 tile halves that.  One workgroup of 8 wavefronts
+// (2 per SIMD) per tile, each wavefront a 128x64 quarter-column of 16x16
+// MFMA blocks (8 x 4, 128 accumulators).
+//
+// K is staged through LDS in chunks of 32, double buffered (2 x 64 KiB):
+// the global loads of chunk c+1 are in flight in registers while chunk c is
+// multiplied.  Rows are stored unpadded and XOR-swizzled by 16-byte slot so
+// that both the row-wise staging writes and the MFMA fragment reads (lane i
+// reads row i of a 16-row block) hit distinct banks:
+//   FP32  : 32 floats = 8 slots per row, slot' = slot ^ (row & 7);
+//   BF16x3: 32 bf16 = 4 slots per row (hi and lo arrays),
+//           slot' = slot ^ ((row >> 1) & 3).
+// BF16x3 splits x = hi + lo exactly as the 128-tile form does and
+// accumulates hi*hi + hi*lo + lo*hi in FP32.
+//
+// Tile order: workgroups are dispatched round-robin over the 8 XCDs, each
+// with its own L2; the block index is remapped so that every XCD takes a
+// contiguous run of the column-by-column tile list, whose neighbours share
+// the column operand block and nearby row blocks in that XCD's L2.
+//
+// Reference counterpart: the trailing SYRK-as-GEMM updates of the blocked
+// drivers (submission.py:5163-5184, :2529-2534), run there on tensor cores.
+#include "chol_common.h"
+
+namespace chol {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BT = 256;                  // tile edge
+constexpr int NT = 512;                  // threads (8 wavefronts)
+constexpr int KC = 32;                   // K chunk
+constexpr int PER = BT * KC / 4 / NT;    // float4 per thread per operand and chunk (4)
+constexpr int STAGE = 2 * BT * KC;       // floats per stage: A then B (64 KiB)
+constexpr int WM = 128, WN = 64;         // per-wave tile
+constexpr int FM = WM / 16, FN = WN / 16;
+
+template <bool VEC>
+__device__ __forceinline__ void load_chunk(float4 (&reg)[PER], const float* base, int64_t ld, int row0,
+                                           int rlim, int k, int tid) {
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int c = tid + u * NT;
+    const int r = c >> 3;
+    const int q = (c & 7) * 4;
+    const int gr = row0 + r;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (gr < rlim) {
+      const float* src = base + (int64_t)gr * ld + k + q;
+      if constexpr (VEC) {
+        v = gld4(src);
+      } else {
+        v.x = gld(src);
+        v.y = gld(src + 1);
+        v.z = gld(src + 2);
+        v.w = gld(src + 3);
+      }
+    }
+    reg[u] = v;
+  }
+}
+
+// FP32 operand: row r at r*32 floats, 16-byte slot q at (q ^ (r & 7)) * 4.
+__device__ __forceinline__ void put_f32(float* __restrict__ s, const float4 (&reg)[PER], int tid) {
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int c = tid + u * NT;
+    const int r = c >> 3;
+    const int q = c & 7;
+    *reinterpret_cast<float4*>(s + r * KC + ((q ^ (r & 7)) << 2)) = reg[u];
+  }
+}
+
+// BF16 operand (hi at s, lo at s + BT*KC): row r at r*32 elements, 16-byte
+// slot g (k = 8g..8g+7) at (g ^ ((r >> 1) & 3)) * 8; a float4 (k = 4q..4q+3)
+// fills half a slot.
+__device__ __forceinline__ void put_split(__bf16* __restrict__ s, const float4 (&reg)[PER], int tid) {
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int c = tid + u * NT;
+    const int r = c >> 3;
+    const int q = c & 7;
+    const float4 v = reg[u];
+    const __bf16 h0 = (__bf16)v.x, h1 = (__bf16)v.y, h2 = (__bf16)v.z, h3 = (__bf16)v.w;
+    const bf16x4 hi = {h0, h1, h2, h3};
+    const bf16x4 lo = {(__bf16)(v.x - (float)h0), (__bf16)(v.y - (float)h1), (__bf16)(v.z - (float)h2),
+                       (__bf16)(v.w - (float)h3)};
+    const int off = r * KC + (((q >> 1) ^ ((r >> 1) & 3)) << 3) + ((q & 1) << 2);
+    *reinterpret_cast<bf16x4*>(s + off) = hi;
+    *reinterpret_cast<bf16x4*>(s + BT * KC + off) = lo;
+  }
+}
+
+template <bool VEC, bool SPLIT>
+__global__ __launch_bounds__(NT) void k_syrk256(Batch p, int col0, int col1, int k0, int kl) {
+  __shared__ __align__(16) float lds[2 * STAGE];  // 128 KiB
+
+  const int64_t b = blockIdx.y;
+  const Mat M = mat_at(p, b);
+  const int c1 = col1 < 0 ? M.n : col1;
+  if (c1 <= col0) return;
+  const int TI = (M.n - col0 + BT - 1) / BT;
+  const int TJ = (c1 - col0 + BT - 1) / BT;
+  const int T = (int)((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2);
+  // XCD-contiguous remap of the tile index (a bijection of [0, T)); grid.x
+  // covers the largest matrix, so ids at or past this matrix's T exit.
+  const int bid = blockIdx.x;
+  if (bid >= T) return;
+  const int xcd = bid & 7, q8 = T >> 3, r8 = T & 7;
+  const int idx = xcd * q8 + (xcd < r8 ? xcd : r8) + (bid >> 3);
+  int ti, tj;
+  decode_tile(idx, TI, ti, tj);
+  const int i0 = col0 + BT * ti;
+  const int j0 = col0 + BT * tj;
+  float* L = M.L;
+  const int64_t ld = M.ld;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 2) * WM;
+  const int wn = (wave & 3) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+
+  floatx4 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int c = 0; c < FN; ++c) acc[a][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  float4 ra[PER], rb[PER];
+  auto stage = [&](int buf) {
+    float* s = lds + buf * STAGE;
+    if constexpr (SPLIT) {
+      __bf16* h = reinterpret_cast<__bf16*>(s);
+      put_split(h, ra, tid);               // A hi, A lo
+      put_split(h + 2 * BT * KC, rb, tid); // B hi, B lo
+    } else {
+      put_f32(s, ra, tid);
+      put_f32(s + BT * KC, rb, tid);
+    }
+  };
+
+  load_chunk<VEC>(ra, L, ld, i0, M.n, k0, tid);
+  load_chunk<VEC>(rb, L, ld, j0, c1, k0, tid);
+  stage(0);
+  __syncthreads();
+
+  const int nch = kl / KC;
+  for (int ch = 0; ch < nch; ++ch) {
+    const int cur = ch & 1;
+    const bool more = ch + 1 < nch;
+    if (more) {
+      load_chunk<VEC>(ra, L, ld, i0, M.n, k0 + (ch + 1) * KC, tid);
+      load_chunk<VEC>(rb, L, ld, j0, c1, k0 + (ch + 1) * KC, tid);
+    }
+    const float* s = lds + cur * STAGE;
+    if constexpr (SPLIT) {
+      const __bf16* Ah = reinterpret_cast<const __bf16*>(s);
+      const __bf16* Al = Ah + BT * KC;
+      const __bf16* Bh = Ah + 2 * BT * KC;
+      const __bf16* Bl = Ah + 3 * BT * KC;
+      const int sl = (lg ^ ((li >> 1) & 3)) << 3;  // swizzled slot of this lane
+      bf16x8 bh[FN], bl[FN];
+#pragma unroll
+      for (int c = 0; c < FN; ++c) {
+        const int o = (wn + c * 16 + li) * KC + sl;
+        bh[c] = *reinterpret_cast<const bf16x8*>(Bh + o);
+        bl[c] = *reinterpret_cast<const bf16x8*>(Bl + o);
+      }
+#pragma unroll
+      for (int a = 0; a < FM; ++a) {
+        const int o = (wm + a * 16 + li) * KC + sl;
+        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(Ah + o);
+        const bf16x8 al = *reinterpret_cast<const bf16x8*>(Al + o);
+#pragma unroll
+        for (int c = 0; c < FN; ++c) {
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[c], acc[a][c], 0, 0, 0);
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[c], acc[a][c], 0, 0, 0);
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[c], acc[a][c], 0, 0, 0);
+        }
+      }
+    } else {
+      const float* As = s;
+      const float* Bs = s + BT * KC;
+#pragma unroll
+      for (int ks = 0; ks < KC; ks += 16) {
+        const int sl = (((ks >> 2) + lg) ^ (li & 7)) << 2;  // swizzled slot of this lane
+        float4 fb[FN];
+#pragma unroll
+        for (int c = 0; c < FN; ++c) fb[c] = *reinterpret_cast<const float4*>(Bs + (wn + c * 16 + li) * KC + sl);
+#pragma unroll
+        for (int a = 0; a < FM; ++a) {
+          const float4 fa = *reinterpret_cast<const float4*>(As + (wm + a * 16 + li) * KC + sl);
+#pragma unroll
+          for (int c = 0; c < FN; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.x, fb[c].x, acc[a][c], 0, 0, 0);
+#pragma unroll
+          for (int c = 0; c < FN; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.y, fb[c].y, acc[a][c], 0, 0, 0);
+#pragma unroll
+          for (int c = 0; c < FN; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.z, fb[c].z, acc[a][c], 0, 0, 0);
+#pragma unroll
+          for (int c = 0; c < FN; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa.w, fb[c].w, acc[a][c], 0, 0, 0);
+        }
+      }
+    }
+    if (more) stage(cur ^ 1);
+    __syncthreads();
+  }
+
+  // C -= D on the lower part (j <= i), i < n, j < c1.  Lane holds
+  // D[4*lg + v][li] of each 16x16 block.  Two rounds of 64 elements: every
+  // load of a round is issued before its stores (a masked element reads
+  // C at the tile's own corner, always valid, and is not stored); the
+  // addresses are recomputed for the stores rather than held.
+  const int jl = j0 + wn + li;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    float old[FM / 2][FN][4];
+#pragma unroll
+    for (int a = 0; a < FM / 2; ++a)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wm + (h * (FM / 2) + a) * 16 + 4 * lg + v;
+        const float* row = L + (int64_t)(i < M.n ? i : i0) * ld;
+#pragma unroll
+        for (int c = 0; c < FN; ++c) {
+          const int j = jl + c * 16;
+          const bool m = i < M.n && j < c1 && j <= i;
+          old[a][c][v] = gld(m ? row + j : L + (int64_t)i0 * ld + j0);
+        }
+      }
+#pragma unroll
+    for (int a = 0; a < FM / 2; ++a)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wm + (h * (FM / 2) + a) * 16 + 4 * lg + v;
+        float* row = L + (int64_t)i * ld;
+#pragma unroll
+        for (int c = 0; c < FN; ++c) {
+          const int j = jl + c * 16;
+          if (i < M.n && j < c1 && j <= i) gst(row + j, old[a][c][v] - acc[h * (FM / 2) + a][c][v]);
+        }
+      }
+  }
+}
+
+}  // namespace
+
+int64_t syrk256_tiles(const Batch& p, int col0, int col1) {
+  const int TI = (p.n - col0 + BT - 1) / BT;
+  const int TJ = ((col1 < 0 ? p.n : col1) - col0 + BT - 1) / BT;
+  return ((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) * p.batch;
+}
+
+int launch_syrk256(const Batch& p, int col0, int col1, int k0, int kl, bool split, bool vec, hipStream_t s) {
+  const int TI = (p.n - col0 + BT - 1) / BT;
+  const int TJ = ((col1 < 0 ? p.n : col1) - col0 + BT - 1) / BT;
+  const int64_t tiles = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
+  dim3 grid((unsigned)tiles, (unsigned)p.batch);
+  if (split) {
+    if (vec)
+      hipLaunchKernelGGL((k_syrk256<true, true>), grid, dim3(NT), 0, s, p, col0, col1, k0, kl);
+    else
+      hipLaunchKernelGGL((k_syrk256<false, true>), grid, dim3(NT), 0, s, p, col0, col1, k0, kl);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL((k_syrk256<true, false>), grid, dim3(NT), 0, s, p, col0, col1, k0, kl);
+    else
+      hipLaunchKernelGGL((k_syrk256<false, false>), grid, dim3(NT), 0, s, p, col0, col1, k0, kl);
+  }
+  return launch_status("k_syrk256");
+}
+
+}  // namespace chol
