@@ -92,9 +92,10 @@ static int side_stream(SideStream*& out) {
   return 0;
 }
 
-// Blocked right-looking factorisation with one outer block of look-ahead.
-// Within outer block [K0, K0+W) the 64-wide steps run on the caller's
-// stream s.  The block's trailing update (K = W) is split: the next outer
+// Blocked factorisation with one outer block of look-ahead.  Within outer
+// block [K0, K0+W) the 128-wide steps run on the caller's stream s.  Across
+// outer blocks the update is right-looking: the block's trailing update
+// (K = W) is split: the next outer
 // block's columns (narrow, needed next) stay on s; the columns beyond
 // (bulk) go to the side stream, where they overlap the next block's chain
 // of small launches.  A narrow update first waits for the previous bulk
@@ -108,13 +109,19 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
   bool bulk_pending = false;
   for (int K0 = 0; K0 < n; K0 += nbo) {
     const int W = (n - K0) < nbo ? (n - K0) : nbo;
+    // Inside the block: left-looking when the strip is tall enough that its
+    // updates are throughput work (each column block is brought up to date
+    // with the block's earlier columns, K = k0 - K0, just before its
+    // diagonal step: the strip is read and written once per column block);
+    // right-looking (K = 128 updates of all later columns of the block, one
+    // short launch each) when they are latency work.
+    const bool left = (int64_t)((n - K0) / kNB) * p.batch >= 256;
     for (int k0 = K0; k0 < K0 + W; k0 += kNB) {
+      if (left && k0 > K0 && (rc = launch_syrk(p, k0, k0 + kNB, K0, k0 - K0, split, s))) return rc;
       if ((rc = launch_diag_factor(p, k0, winv, s))) return rc;
       if (k0 + kNB >= n) break;
       if ((rc = launch_panel(p, k0, winv, s))) return rc;
-      if (k0 + kNB < K0 + W) {
-        if ((rc = launch_syrk(p, k0 + kNB, K0 + W, k0, kNB, split, s))) return rc;
-      }
+      if (!left && k0 + kNB < K0 + W && (rc = launch_syrk(p, k0 + kNB, K0 + W, k0, kNB, split, s))) return rc;
     }
     const int c0 = K0 + W;  // first column after this outer block
     if (c0 >= n) break;

@@ -4,8 +4,9 @@
 // One workgroup of 4 wavefronts per matrix, the block resident in LDS:
 //   1. all waves stage the (already updated) block with coalesced row loads;
 //      rows past n are the identity;
-//   2. wavefront 0 factors the top-left 64x64 block D1 in registers
-//      (reg_factor: lane = row, pivot column exchanged in the wave);
+//   2. wavefront 0 factors the top-left 64x64 block D1 (factor64: 16-column
+//      panels in registers, lane = row, pivots broadcast with v_readlane,
+//      MFMA updates between panels);
 //   3. W1 = D1^-1 by 16x16 blocks: the 4 wavefronts invert the four diagonal
 //      16x16 blocks at once (lane = column, forward substitution), then
 //      block rows 1..3 in turn, W_ij = -W_ii sum_{k=j}^{i-1} L_ik W_kj;
@@ -56,78 +57,74 @@ __device__ __forceinline__ floatx4 mma_nt(const float* X, const float* Y, floatx
   return acc;
 }
 
-// Wavefront 0 factors the 64x64 block of Ls at (d, d) as two 32-pivot
-// register chains around one MFMA update:
-//   1. lane = row of the 64x32 left panel: reg_factor<32> yields L11 and,
-//      for rows 32..63, L21 (the panel's own triangular solve);
-//   2. A22 -= L21 L21^T (32x32, K = 32) on v_mfma_f32_16x16x4_f32;
-//   3. lanes 0..31 = rows of A22: reg_factor<32> yields L22.
-// A pivot step's cost is one LDS round trip plus the row updates, about
-// half of which vanish with 32-wide rows; one 64-wide chain costs about
-// twice as much.  Returns the 1-based first failing pivot (0 if none).
-__device__ __forceinline__ int factor64(float* Ls, float* cb, int d, int lane) {
-  constexpr int Q = H / 2;  // 32
+// Wavefront 0 factors the 64x64 block of Ls at (d, d), blocked by PW
+// columns: for each column panel [c0, c0+PW), lane = row,
+//   1. reg_factor_rl<PW> on rows c0..d+63 yields the panel's diagonal factor
+//      and, for the rows below it, its part of L (the panel's own
+//      triangular solve), broadcasting pivot columns with v_readlane;
+//   2. the lower 16x16 tiles of the trailing block take the panel's rank-PW
+//      update on v_mfma_f32_16x16x4_f32.
+// A pivot step then costs a few dependent VALU operations plus
+// 2 (PW-1-k) issue slots, where a 64-wide register chain spends an LDS
+// round trip and 63-k row updates.  Returns the 1-based first failing
+// pivot (0 if none).
+template <int PW>
+__device__ __forceinline__ int factor64(float* Ls, int d, int lane) {
   const int li = lane & 15, lg = lane >> 4;
-  int bad;
-  {
-    float a[Q];
-    float* row = &Ls[(d + lane) * P + d];
+  int bad = 0;
 #pragma unroll
-    for (int j = 0; j < Q; j += 4) {
+  for (int s = 0; s < H / PW; ++s) {
+    const int c0 = d + PW * s;
+    float* row = &Ls[(d + lane) * P + c0];
+    float a[PW];
+#pragma unroll
+    for (int j = 0; j < PW; j += 4) {
       const float4 v = *reinterpret_cast<const float4*>(row + j);
       a[j] = v.x;
       a[j + 1] = v.y;
       a[j + 2] = v.z;
       a[j + 3] = v.w;
     }
-    bad = reg_factor<Q>(a, lane, cb, true);
+    const int b = reg_factor_rl<PW>(a, lane, PW * s);
+    if (bad == 0 && b != 0) bad = PW * s + b;
+    if (lane >= PW * s) {
 #pragma unroll
-    for (int j = 0; j < Q; j += 4) *reinterpret_cast<float4*>(row + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
-  }
-  wave_lds_sync();
-  {
-    // lower 16x16 tiles (0,0), (1,0), (1,1) of A22; the update writes only
-    // columns d+32.., the products read only columns d..d+31
-    floatx4 acc[3];
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      const int ti = (t + 1) / 2, tj = t / 2;
-      acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kk = 0; kk < Q; kk += 16)
-        acc[t] = mma_nt(&Ls[(d + Q + 16 * ti) * P + d + kk], &Ls[(d + Q + 16 * tj) * P + d + kk], acc[t], li, lg);
-    }
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      const int ti = (t + 1) / 2, tj = t / 2;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
-        if (ti != tj || j <= i) Ls[(d + Q + i) * P + d + Q + j] -= acc[t][v];
-      }
-    }
-  }
-  wave_lds_sync();
-  {
-    // lanes 32..63 mirror rows 0..31 of A22 and do not write
-    const int r = lane & (Q - 1);
-    float a[Q];
-    float* row = &Ls[(d + Q + r) * P + d + Q];
-#pragma unroll
-    for (int j = 0; j < Q; j += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(row + j);
-      a[j] = v.x;
-      a[j + 1] = v.y;
-      a[j + 2] = v.z;
-      a[j + 3] = v.w;
-    }
-    const int bad2 = reg_factor<Q>(a, r, cb, lane < Q);
-    if (lane < Q) {
-#pragma unroll
-      for (int j = 0; j < Q; j += 4)
+      for (int j = 0; j < PW; j += 4)
         *reinterpret_cast<float4*>(row + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
     }
-    if (bad == 0 && bad2 != 0) bad = Q + bad2;
+    wave_lds_sync();
+    // trailing update: lower 16x16 tiles (ti >= tj) of rows / columns
+    // [PW (s+1), 64); reads only panel columns, writes only later ones
+    const int t0 = PW * (s + 1) / 16;
+    if (t0 < 4) {
+      floatx4 acc[10];
+      int nt = 0;
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj <= ti; ++tj) {
+          if (tj < t0) continue;
+          floatx4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kk = 0; kk < PW; kk += 16)
+            c = mma_nt(&Ls[(d + 16 * ti) * P + c0 + kk], &Ls[(d + 16 * tj) * P + c0 + kk], c, li, lg);
+          acc[nt++] = c;
+        }
+      nt = 0;
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj <= ti; ++tj) {
+          if (tj < t0) continue;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
+            if (ti != tj || j <= i) Ls[(d + i) * P + d + j] -= acc[nt][v];
+          }
+          ++nt;
+        }
+      wave_lds_sync();
+    }
   }
   return bad;
 }
@@ -180,12 +177,10 @@ __device__ __forceinline__ void inverse64(const float* Ls, float* Ws, float (*Ts
   }
 }
 
-// LDS of the kernel, in this order on purpose: the pivot-column buffer and
-// the staging tiles sit below 64 KiB, where every access of the register
-// factorisation encodes its address as an immediate offset (no per-read
-// address register).
+// LDS of the kernel: the staging tiles first, then the block and its
+// inverse (Ls starts below 64 KiB, so most of its accesses encode their
+// offsets as immediates).
 struct __align__(16) DiagLds {
-  float cb[H];
   float Ts[4][16 * TP];  // per-wave 16x16 staging
   float Ls[NB * P];
   float Ws[NB * P];
@@ -193,7 +188,6 @@ struct __align__(16) DiagLds {
 
 __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv) {
   __shared__ DiagLds sm;
-  float* const cb = sm.cb;
   float(*const Ts)[16 * TP] = sm.Ts;
   float* const Ls = sm.Ls;
   float* const Ws = sm.Ws;
@@ -271,7 +265,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 
   // ---- 2-3. D1 and W1 ----
   int bad = 0;
-  if (wave == 0) bad = factor64(Ls, cb, 0, lane);
+  if (wave == 0) bad = factor64<16>(Ls, 0, lane);
   __syncthreads();
   inverse64(Ls, Ws, Ts, 0, wave, lane, li, lg);
 
@@ -314,7 +308,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 
   // ---- 5. D2 and W2 ----
   if (wave == 0) {
-    const int bad2 = factor64(Ls, cb, H, lane);
+    const int bad2 = factor64<16>(Ls, H, lane);
     if (bad == 0 && bad2 != 0) bad = H + bad2;
   }
   __syncthreads();

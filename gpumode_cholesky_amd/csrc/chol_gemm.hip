@@ -553,10 +553,12 @@ void syrk_launch(const Batch& p, int col0, int col1, int k0, int kl, bool vec, h
 int launch_panel(const Batch& p, int k0, const float* winv, hipStream_t s) {
   const int rows = p.n - (k0 + kNB);
   if (rows <= 0) return 0;
-  const bool small = (int64_t)((rows + 127) / 128) * p.batch < kFillWGs;
+  // One-shot 32-row tiles (W, 64 KiB, and the rows staged in one pass)
+  // while four times the 128-row tile count still fits the GPU about once;
+  // beyond that the pipelined 128-row tiles, which stage W once per 128 rows.
+  const bool small = (int64_t)((rows + 127) / 128) * p.batch <= 64;
   const bool vec = vec_ok(p);
   if (small) {
-    // one-shot 32-row tiles: W (64 KiB) and the rows staged in one pass
     dim3 grid((unsigned)((rows + 31) / 32), (unsigned)p.batch);
     if (vec)
       hipLaunchKernelGGL((k_panel_once<32, true>), grid, dim3(kThreads), 0, s, p, k0, winv);
