@@ -181,30 +181,33 @@ __device__ __forceinline__ float readlane_f(float x, int l) {
 // diagonal block's rows r0..r0+NP-1 are lanes r0..r0+NP-1 (lane = row,
 // r0 wave-uniform), and c[j] = column k of row r0+j is v_readlane of lane
 // r0+j's a[k].  Rows below the block (r >= r0+NP) come out as the panel
-// L[r][k]; lanes r < r0 compute values that the caller discards.  With no
-// LDS round trip a pivot step costs a few dependent VALU operations on the
-// critical path plus 2 (NP-1-k) issue slots.  Same arithmetic as
-// reg_factor, operation for operation.
+// L[r][k]; lanes r < r0 compute values that the caller discards.
+//
+// Critical path of a pivot step: the update of column k+1, its read-back,
+// v_rcp_f32 of the pivot and one multiply, so the multiplier is
+// t = a[k] * rcp(c[k]) (not (a[k] rsq) rsq) and is applied unmasked: the
+// entries it touches in rows at or above the pivot lie above the diagonal
+// and are zeroed when their own pivot comes.  L[r][k] = a[k] * rsq(c[k])
+// for r >= k (the diagonal c[k] rsq(c[k]), within ~1 ulp of sqrt) is off
+// that path.  A non-positive pivot yields NaN from there on; the return
+// value is the 1-based first failing pivot (0 if none).
 template <int NP>
 __device__ __forceinline__ int reg_factor_rl(float (&a)[NP], int r, int r0) {
   int bad = 0;
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
     // the whole pivot column first: the reads go to distinct scalar
-    // registers while the reciprocal square root is in flight
+    // registers while the reciprocal is in flight
     float c[NP];
 #pragma unroll
     for (int j = k; j < NP; ++j) c[j] = readlane_f(a[k], r0 + j);
     const float piv = c[k];
     if (bad == 0 && !(piv > 0.f)) bad = k + 1;
-    const float is = __builtin_amdgcn_rsqf(piv);
-    const float lk = a[k] * is;
-    const float t = (r > r0 + k) ? lk * is : 0.f;
+    const float t = a[k] * __builtin_amdgcn_rcpf(piv);
 #pragma unroll
     for (int j = k + 1; j < NP; ++j) a[j] = fmaf(-t, c[j], a[j]);
-    const float s0 = piv * is;
-    const float s = fmaf(0.5f * is, fmaf(-s0, s0, piv), s0);
-    a[k] = (r > r0 + k) ? lk : ((r == r0 + k) ? s : 0.f);
+    const float lk = a[k] * __builtin_amdgcn_rsqf(piv);
+    a[k] = (r >= r0 + k) ? lk : 0.f;
   }
   return bad;
 }

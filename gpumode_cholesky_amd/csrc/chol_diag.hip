@@ -20,6 +20,7 @@
 // Reference counterpart: e62_diag128 (submission.py:4694-4997), which
 // factors a 128x128 block and builds its inverse in one CTA.
 #include "chol_common.h"
+#include "chol_block.h"
 
 namespace chol {
 namespace {
@@ -55,78 +56,6 @@ __device__ __forceinline__ floatx4 mma_nt(const float* X, const float* Y, floatx
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.z, yb.z, acc, 0, 0, 0);
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.w, yb.w, acc, 0, 0, 0);
   return acc;
-}
-
-// Wavefront 0 factors the 64x64 block of Ls at (d, d), blocked by PW
-// columns: for each column panel [c0, c0+PW), lane = row,
-//   1. reg_factor_rl<PW> on rows c0..d+63 yields the panel's diagonal factor
-//      and, for the rows below it, its part of L (the panel's own
-//      triangular solve), broadcasting pivot columns with v_readlane;
-//   2. the lower 16x16 tiles of the trailing block take the panel's rank-PW
-//      update on v_mfma_f32_16x16x4_f32.
-// A pivot step then costs a few dependent VALU operations plus
-// 2 (PW-1-k) issue slots, where a 64-wide register chain spends an LDS
-// round trip and 63-k row updates.  Returns the 1-based first failing
-// pivot (0 if none).
-template <int PW>
-__device__ __forceinline__ int factor64(float* Ls, int d, int lane) {
-  const int li = lane & 15, lg = lane >> 4;
-  int bad = 0;
-#pragma unroll
-  for (int s = 0; s < H / PW; ++s) {
-    const int c0 = d + PW * s;
-    float* row = &Ls[(d + lane) * P + c0];
-    float a[PW];
-#pragma unroll
-    for (int j = 0; j < PW; j += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(row + j);
-      a[j] = v.x;
-      a[j + 1] = v.y;
-      a[j + 2] = v.z;
-      a[j + 3] = v.w;
-    }
-    const int b = reg_factor_rl<PW>(a, lane, PW * s);
-    if (bad == 0 && b != 0) bad = PW * s + b;
-    if (lane >= PW * s) {
-#pragma unroll
-      for (int j = 0; j < PW; j += 4)
-        *reinterpret_cast<float4*>(row + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
-    }
-    wave_lds_sync();
-    // trailing update: lower 16x16 tiles (ti >= tj) of rows / columns
-    // [PW (s+1), 64); reads only panel columns, writes only later ones
-    const int t0 = PW * (s + 1) / 16;
-    if (t0 < 4) {
-      floatx4 acc[10];
-      int nt = 0;
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-        for (int tj = 0; tj <= ti; ++tj) {
-          if (tj < t0) continue;
-          floatx4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-          for (int kk = 0; kk < PW; kk += 16)
-            c = mma_nt(&Ls[(d + 16 * ti) * P + c0 + kk], &Ls[(d + 16 * tj) * P + c0 + kk], c, li, lg);
-          acc[nt++] = c;
-        }
-      nt = 0;
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-        for (int tj = 0; tj <= ti; ++tj) {
-          if (tj < t0) continue;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
-            if (ti != tj || j <= i) Ls[(d + i) * P + d + j] -= acc[nt][v];
-          }
-          ++nt;
-        }
-      wave_lds_sync();
-    }
-  }
-  return bad;
 }
 
 // Ws(d..d+64, d..d+64) = inverse of the lower-triangular Ls(d.., d..) block,
@@ -265,7 +194,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 
   // ---- 2-3. D1 and W1 ----
   int bad = 0;
-  if (wave == 0) bad = factor64<16>(Ls, 0, lane);
+  if (wave == 0) bad = factor64<16, P>(Ls, 0, lane);
   __syncthreads();
   inverse64(Ls, Ws, Ts, 0, wave, lane, li, lg);
 
@@ -308,7 +237,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 
   // ---- 5. D2 and W2 ----
   if (wave == 0) {
-    const int bad2 = factor64<16>(Ls, H, lane);
+    const int bad2 = factor64<16, P>(Ls, H, lane);
     if (bad == 0 && bad2 != 0) bad = H + bad2;
   }
   __syncthreads();

@@ -9,8 +9,8 @@
 //
 // Per 16-wide panel p (one s_barrier between phases):
 //   1. wavefront 0, lanes 0..15: factor the 16x16 diagonal block in
-//      registers (pivot column exchanged through LDS inside the wave) and
-//      form its inverse W = L11^-1;
+//      registers (pivot columns broadcast with v_readlane, reg_factor_rl)
+//      and form its inverse W = L11^-1;
 //   2. all 4 wavefronts: L21 = A21 W^T, one 16-row block per MFMA tile
 //      (v_mfma_f32_16x16x4_f32, K = 16);
 //   3. all 4 wavefronts: trailing update C -= L21 L21^T over the lower 16x16
@@ -44,7 +44,6 @@ __device__ __forceinline__ void lds_factor(const Batch& p, int64_t b) {
   constexpr int WP = kPB + 4;
   __shared__ __align__(16) float S[kFloats];
   __shared__ __align__(16) float Wt[kPB * WP];  // W = L11^-1, row-major
-  __shared__ __align__(16) float cb[kPB];
   __shared__ int bad_s;
 
   const int tid = threadIdx.x;
@@ -90,7 +89,7 @@ __device__ __forceinline__ void lds_factor(const Batch& p, int64_t b) {
         a[j + 2] = v.z;
         a[j + 3] = v.w;
       }
-      const int bad = reg_factor<kPB>(a, r, cb, lane < kPB);
+      const int bad = reg_factor_rl<kPB>(a, r, 0);
       if (lane < kPB) {
         // Chunks j <= r only; their entries past r are zeros that land in
         // the row's own padding.

@@ -18,6 +18,7 @@
 // Only the lower triangle of A is loaded; the upper part of each lane's row
 // never feeds another lane, and is zeroed as the loop passes it.
 #include "chol_common.h"
+#include "chol_block.h"
 
 namespace chol {
 namespace {
@@ -58,8 +59,10 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
   constexpr int MPW = kWave / NP;  // matrices per wavefront
   constexpr int PITCH = NP + 4;    // padded row / column stride (bank spread)
   __shared__ __align__(16) float colbuf[kWavesPerBlock][MPW * PITCH];
-  // VEC: per-wave staging of its MPW matrices, one padded row per lane.
-  __shared__ __align__(16) float stage[VEC ? kWavesPerBlock : 1][VEC ? kWave * PITCH : 4];
+  // VEC, and always for NP = 64: per-wave staging of its MPW matrices, one
+  // padded row per lane.
+  constexpr bool STAGED = VEC || NP == 64;
+  __shared__ __align__(16) float stage[STAGED ? kWavesPerBlock : 1][STAGED ? kWave * PITCH : 4];
 
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
@@ -89,30 +92,59 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
       *reinterpret_cast<float4*>(st + row * PITCH + col) = v;
     }
     wave_lds_sync();
+    if constexpr (NP != 64) {
 #pragma unroll
-    for (int j = 0; j < NP; j += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(st + lane * PITCH + j);
-      a[j] = v.x;
-      a[j + 1] = v.y;
-      a[j + 2] = v.z;
-      a[j + 3] = v.w;
+      for (int j = 0; j < NP; j += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(st + lane * PITCH + j);
+        a[j] = v.x;
+        a[j + 1] = v.y;
+        a[j + 2] = v.z;
+        a[j + 3] = v.w;
+      }
     }
   } else {
     slot_rows<NP>(p, b, live, r, a);
+    if constexpr (NP == 64) {
+      float* st = stage[wave];
+#pragma unroll
+      for (int j = 0; j < NP; j += 4)
+        *reinterpret_cast<float4*>(st + lane * PITCH + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+      wave_lds_sync();
+    }
   }
 
-  float* cb = &colbuf[wave][m * PITCH];
-  const int32_t bad = reg_factor<NP>(a, r, cb, true);
+  int32_t bad;
+  if constexpr (NP == 64) {
+    // one matrix per wave, staged in LDS: 16-column register panels with
+    // MFMA trailing updates (the factor overwrites the staged rows)
+    bad = factor64<16, PITCH>(stage[wave], 0, lane);
+    wave_lds_sync();
+    if constexpr (!VEC) {
+#pragma unroll
+      for (int j = 0; j < NP; j += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(stage[wave] + lane * PITCH + j);
+        a[j] = v.x;
+        a[j + 1] = v.y;
+        a[j + 2] = v.z;
+        a[j + 3] = v.w;
+      }
+    }
+  } else {
+    float* cb = &colbuf[wave][m * PITCH];
+    bad = reg_factor<NP>(a, r, cb, true);
+  }
   if (live && p.info && r == 0) p.info[VEC ? b : mat_at(p, b).slot] = bad;
 
   if constexpr (VEC) {
     // Rows back through LDS, then coalesced float4 stores (zeros above the
     // diagonal included).
     float* st = stage[wave];
+    if constexpr (NP != 64) {
 #pragma unroll
-    for (int j = 0; j < NP; j += 4)
-      *reinterpret_cast<float4*>(st + lane * PITCH + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
-    wave_lds_sync();
+      for (int j = 0; j < NP; j += 4)
+        *reinterpret_cast<float4*>(st + lane * PITCH + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+      wave_lds_sync();
+    }
     float* dst = p.L + b0 * (int64_t)NP * NP;
 #pragma unroll
     for (int u = 0; u < NP / 4; ++u) {
@@ -136,6 +168,8 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small_queue(Bat
   constexpr int MPW = kWave / NP;
   constexpr int PITCH = NP + 4;
   __shared__ __align__(16) float colbuf[kWavesPerBlock][MPW * PITCH];
+  // NP = 64: the same LDS-staged blocked factorisation as k_potrf_small
+  __shared__ __align__(16) float stage[NP == 64 ? kWavesPerBlock : 1][NP == 64 ? kWave * PITCH : 4];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int m = lane / NP;
@@ -151,7 +185,27 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small_queue(Bat
     const bool live = idx + m < count;
     float a[NP];
     slot_rows<NP>(p, b, live, r, a);
-    const int32_t bad = reg_factor<NP>(a, r, cb, true);
+    int32_t bad;
+    if constexpr (NP == 64) {
+      float* st = stage[wave];
+#pragma unroll
+      for (int j = 0; j < NP; j += 4)
+        *reinterpret_cast<float4*>(st + lane * PITCH + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+      wave_lds_sync();
+      bad = factor64<16, PITCH>(st, 0, lane);
+      wave_lds_sync();
+#pragma unroll
+      for (int j = 0; j < NP; j += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(st + lane * PITCH + j);
+        a[j] = v.x;
+        a[j + 1] = v.y;
+        a[j + 2] = v.z;
+        a[j + 3] = v.w;
+      }
+      wave_lds_sync();  // the next slot's rows overwrite the stage
+    } else {
+      bad = reg_factor<NP>(a, r, cb, true);
+    }
     if (live && p.info && r == 0) p.info[mat_at(p, b).slot] = bad;
     slot_store<NP>(p, b, live, r, a);
   }
