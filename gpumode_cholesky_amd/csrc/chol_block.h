@@ -95,4 +95,67 @@ __device__ __forceinline__ int factor64(float* Ls, int d, int lane) {
   return bad;
 }
 
+// Two 32x32 matrices per wavefront, staged in LDS (lane l holds row l % 32 of
+// matrix l / 32 at st + l*P), factored as two 16-column panels:
+//   1. reg_factor<16> on rows 0..31 of each matrix (pivot columns exchanged
+//      through cb, 32 floats per matrix): L11 and L21 of the left panel;
+//   2. A22 -= L21 L21^T for each matrix (one 16x16 tile, K = 16, MFMA);
+//   3. reg_factor<16> on rows 16..31: L22.
+// Against one 32-pivot chain this halves the row updates (each row spans
+// 16 columns per chain, not up to 32) for one MFMA tile per matrix.
+// Returns the lane's matrix's 1-based first failing pivot (0 if none).
+template <int P>
+__device__ __forceinline__ int factor32_pair(float* st, float* cb, int lane) {
+  const int m = lane >> 5, r = lane & 31, li = lane & 15, lg = (lane >> 4) & 3;
+  float* row = st + lane * P;
+  float* cbm = cb + 32 * m;
+  int bad;
+  {
+    float a[16];
+#pragma unroll
+    for (int j = 0; j < 16; j += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(row + j);
+      a[j] = v.x;
+      a[j + 1] = v.y;
+      a[j + 2] = v.z;
+      a[j + 3] = v.w;
+    }
+    bad = reg_factor<16>(a, r, cbm, true);
+#pragma unroll
+    for (int j = 0; j < 16; j += 4) *reinterpret_cast<float4*>(row + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    float* t = st + (32 * q + 16) * P;
+    const floatx4_b c = mma_nt_p<P>(t, t, floatx4_b{0.f, 0.f, 0.f, 0.f}, li, lg);
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (li <= 4 * lg + v) t[(4 * lg + v) * P + 16 + li] -= c[v];
+  }
+  wave_lds_sync();
+  {
+    // rows 16..31 (r1 = r - 16); lanes with r < 16 take no update and do not write
+    const int r1 = r - 16;
+    float b[16];
+#pragma unroll
+    for (int j = 0; j < 16; j += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(row + 16 + j);
+      b[j] = v.x;
+      b[j + 1] = v.y;
+      b[j + 2] = v.z;
+      b[j + 3] = v.w;
+    }
+    const int bad2 = reg_factor<16>(b, r1, cbm, r1 >= 0);
+    if (r1 >= 0) {
+#pragma unroll
+      for (int j = 0; j < 16; j += 4)
+        *reinterpret_cast<float4*>(row + 16 + j) = make_float4(b[j], b[j + 1], b[j + 2], b[j + 3]);
+    }
+    if (bad == 0 && bad2 != 0) bad = 16 + bad2;
+  }
+  wave_lds_sync();
+  return bad;
+}
+
 }  // namespace chol

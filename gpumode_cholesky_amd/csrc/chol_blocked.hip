@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_copy_lower(Batch p) {
        e += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(e / n);
     const int j = (int)(e - (int64_t)i * n);
-    gst(dst + (int64_t)i * M.ld + j, (j <= i) ? gld(src + (int64_t)i * M.ld + j) : 0.f);
+    gst(dst + (int64_t)i * M.ld + j, (j <= i) ? gld_nt(src + (int64_t)i * M.ld + j) : 0.f);
   }
 }
 
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_copy_lower4(Batch p) {
     const int i = (int)(e / n4);
     const int j = 4 * (int)(e - (int64_t)i * n4);
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (j <= i) v = gld4(src + (int64_t)i * p.ld + j);
+    if (j <= i) v = gld4_nt(src + (int64_t)i * p.ld + j);
     if (j + 1 > i) v.y = 0.f;
     if (j + 2 > i) v.z = 0.f;
     if (j + 3 > i) v.w = 0.f;

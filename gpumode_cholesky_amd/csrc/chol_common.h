@@ -68,6 +68,19 @@ __device__ __forceinline__ float4 gld4(const float* q) {
 }
 __device__ __forceinline__ void gst(float* q, float v) { *(gfloat*)q = v; }
 __device__ __forceinline__ void gst4(float* q, float4 v) { *(gfvec4*)q = fvec4{v.x, v.y, v.z, v.w}; }
+// Streaming (non-temporal) forms for data touched once by the kernel: the
+// factor's final stores and the input's single read.  They keep such data
+// from piling up as dirty lines in the XCD's L2, which the end-of-kernel
+// release would otherwise write back after the last wave.
+__device__ __forceinline__ float4 gld4_nt(const float* q) {
+  const fvec4 v = __builtin_nontemporal_load((const gfvec4*)q);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst4_nt(float* q, float4 v) {
+  __builtin_nontemporal_store(fvec4{v.x, v.y, v.z, v.w}, (gfvec4*)q);
+}
+__device__ __forceinline__ void gst_nt(float* q, float v) { __builtin_nontemporal_store(v, (gfloat*)q); }
+__device__ __forceinline__ float gld_nt(const float* q) { return __builtin_nontemporal_load((const gfloat*)q); }
 
 __device__ __forceinline__ Mat mat_at(const Batch& p, int64_t b) {
   if (p.Ap) {

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 #pragma unroll
     for (int u = 0; u < NB / 8; ++u) {
       const int r = (tid >> 5) + 8 * u;
-      stg[u] = gld4(Lb + (int64_t)r * M.ld + (c4 <= r ? c4 : 0));
+      stg[u] = gld4_nt(Lb + (int64_t)r * M.ld + (c4 <= r ? c4 : 0));
     }
 #pragma unroll
     for (int u = 0; u < NB / 8; ++u) {
@@ -286,13 +286,14 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     if (r < nb && c4 <= r) {
       const float4 v = *reinterpret_cast<const float4*>(&Ls[r * P + c4]);
       float* dst = Lb + (int64_t)r * M.ld + c4;
+      // L11 is final: streaming stores
       if (vec && c4 + 3 < nb) {
-        gst4(dst, v);
+        gst4_nt(dst, v);
       } else {
-        gst(dst, v.x);
-        if (c4 + 1 < nb) gst(dst + 1, v.y);
-        if (c4 + 2 < nb) gst(dst + 2, v.z);
-        if (c4 + 3 < nb) gst(dst + 3, v.w);
+        gst_nt(dst, v.x);
+        if (c4 + 1 < nb) gst_nt(dst + 1, v.y);
+        if (c4 + 2 < nb) gst_nt(dst + 2, v.z);
+        if (c4 + 3 < nb) gst_nt(dst + 3, v.w);
       }
     }
   }

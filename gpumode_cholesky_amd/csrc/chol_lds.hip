@@ -60,7 +60,7 @@ __device__ __forceinline__ void lds_factor(const Batch& p, int64_t b) {
     for (int j = lane; j < len; j += 64) {
       float v;
       if (i < n)
-        v = (j <= i) ? gld(A + (int64_t)i * M.ld + j) : 0.f;
+        v = (j <= i) ? gld_nt(A + (int64_t)i * M.ld + j) : 0.f;
       else
         v = (j == i) ? 1.f : 0.f;
       row[j] = v;
@@ -181,7 +181,7 @@ __device__ __forceinline__ void lds_factor(const Batch& p, int64_t b) {
   for (int i = wave; i < n; i += kThreads / 64) {
     const float* row = S + row_off(i);
     float* dst = Lout + (int64_t)i * M.ld;
-    for (int j = lane; j < n; j += 64) gst(dst + j, (j <= i) ? row[j] : 0.f);
+    for (int j = lane; j < n; j += 64) gst_nt(dst + j, (j <= i) ? row[j] : 0.f);
   }
 }
 

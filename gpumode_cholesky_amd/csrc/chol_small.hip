@@ -36,7 +36,7 @@ __device__ __forceinline__ void slot_rows(const Batch& p, int64_t b, bool live, 
   for (int j = 0; j < NP; ++j) {
     float v;
     if (r < n)
-      v = (live && j <= r) ? gld(src + j) : 0.f;
+      v = (live && j <= r) ? gld_nt(src + j) : 0.f;
     else
       v = (j == r) ? 1.f : 0.f;
     a[j] = v;
@@ -51,7 +51,7 @@ __device__ __forceinline__ void slot_store(const Batch& p, int64_t b, bool live,
   float* dst = M.L + (int64_t)r * M.ld;
 #pragma unroll
   for (int j = 0; j < NP; ++j)
-    if (j < M.n) gst(dst + j, a[j]);
+    if (j < M.n) gst_nt(dst + j, a[j]);
 }
 
 template <int NP, bool VEC>
@@ -59,9 +59,9 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
   constexpr int MPW = kWave / NP;  // matrices per wavefront
   constexpr int PITCH = NP + 4;    // padded row / column stride (bank spread)
   __shared__ __align__(16) float colbuf[kWavesPerBlock][MPW * PITCH];
-  // VEC, and always for NP = 64: per-wave staging of its MPW matrices, one
+  // VEC, and always for NP >= 32: per-wave staging of its MPW matrices, one
   // padded row per lane.
-  constexpr bool STAGED = VEC || NP == 64;
+  constexpr bool STAGED = VEC || NP >= 32;
   __shared__ __align__(16) float stage[STAGED ? kWavesPerBlock : 1][STAGED ? kWave * PITCH : 4];
 
   const int lane = threadIdx.x & (kWave - 1);
@@ -88,11 +88,11 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
       const int col = e % NP;
       const int rr = row % NP;
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (b0 + row / NP < p.batch && col <= rr) v = gld4(src + e);
+      if (b0 + row / NP < p.batch && col <= rr) v = gld4_nt(src + e);
       *reinterpret_cast<float4*>(st + row * PITCH + col) = v;
     }
     wave_lds_sync();
-    if constexpr (NP != 64) {
+    if constexpr (NP < 32) {
 #pragma unroll
       for (int j = 0; j < NP; j += 4) {
         const float4 v = *reinterpret_cast<const float4*>(st + lane * PITCH + j);
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
     }
   } else {
     slot_rows<NP>(p, b, live, r, a);
-    if constexpr (NP == 64) {
+    if constexpr (NP >= 32) {
       float* st = stage[wave];
 #pragma unroll
       for (int j = 0; j < NP; j += 4)
@@ -114,10 +114,14 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
   }
 
   int32_t bad;
-  if constexpr (NP == 64) {
-    // one matrix per wave, staged in LDS: 16-column register panels with
-    // MFMA trailing updates (the factor overwrites the staged rows)
-    bad = factor64<16, PITCH>(stage[wave], 0, lane);
+  if constexpr (NP >= 32) {
+    // staged in LDS, 16-column register panels with MFMA trailing updates
+    // (the factor overwrites the staged rows): one matrix per wave for
+    // NP = 64, two for NP = 32
+    if constexpr (NP == 64)
+      bad = factor64<16, PITCH>(stage[wave], 0, lane);
+    else
+      bad = factor32_pair<PITCH>(stage[wave], colbuf[wave], lane);
     wave_lds_sync();
     if constexpr (!VEC) {
 #pragma unroll
@@ -139,7 +143,7 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
     // Rows back through LDS, then coalesced float4 stores (zeros above the
     // diagonal included).
     float* st = stage[wave];
-    if constexpr (NP != 64) {
+    if constexpr (NP < 32) {
 #pragma unroll
       for (int j = 0; j < NP; j += 4)
         *reinterpret_cast<float4*>(st + lane * PITCH + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small(Batch p) 
       const int row = e / NP;
       const int col = e % NP;
       if (b0 + row / NP < p.batch)
-        gst4(dst + e, *reinterpret_cast<const float4*>(st + row * PITCH + col));
+        gst4_nt(dst + e, *reinterpret_cast<const float4*>(st + row * PITCH + col));
     }
   } else {
     slot_store<NP>(p, b, live, r, a);
@@ -168,8 +172,8 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small_queue(Bat
   constexpr int MPW = kWave / NP;
   constexpr int PITCH = NP + 4;
   __shared__ __align__(16) float colbuf[kWavesPerBlock][MPW * PITCH];
-  // NP = 64: the same LDS-staged blocked factorisation as k_potrf_small
-  __shared__ __align__(16) float stage[NP == 64 ? kWavesPerBlock : 1][NP == 64 ? kWave * PITCH : 4];
+  // NP >= 32: the same LDS-staged blocked factorisation as k_potrf_small
+  __shared__ __align__(16) float stage[NP >= 32 ? kWavesPerBlock : 1][NP >= 32 ? kWave * PITCH : 4];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int m = lane / NP;
@@ -186,13 +190,16 @@ __global__ __launch_bounds__(kWavesPerBlock* kWave) void k_potrf_small_queue(Bat
     float a[NP];
     slot_rows<NP>(p, b, live, r, a);
     int32_t bad;
-    if constexpr (NP == 64) {
+    if constexpr (NP >= 32) {
       float* st = stage[wave];
 #pragma unroll
       for (int j = 0; j < NP; j += 4)
         *reinterpret_cast<float4*>(st + lane * PITCH + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
       wave_lds_sync();
-      bad = factor64<16, PITCH>(st, 0, lane);
+      if constexpr (NP == 64)
+        bad = factor64<16, PITCH>(st, 0, lane);
+      else
+        bad = factor32_pair<PITCH>(st, colbuf[wave], lane);
       wave_lds_sync();
 #pragma unroll
       for (int j = 0; j < NP; j += 4) {
