@@ -11,11 +11,12 @@
 //      16x16 blocks at once (lane = column, forward substitution), then
 //      block rows 1..3 in turn, W_ij = -W_ii sum_{k=j}^{i-1} L_ik W_kj;
 //   4. L21 = A21 W1^T and D2 -= L21 L21^T on v_mfma_f32_16x16x4_f32;
-//   5. wavefront 0 factors D2; W2 = D2^-1 as in 3;
-//   6. W21 = -W2 (L21 W1), so W = L11^-1 is complete;
-//   7. all waves store L11 and W with coalesced row stores.
-// Two 64-pivot chains (~7.5 us each at ~110 ns a pivot, measured with an
-// s_memtime probe) dominate; the MFMA steps add ~1-2 us.
+//   5. wavefront 0 factors D2 while waves 1-3 form T1 = L21 W1 and store the
+//      parts that are final (L rows 0..63, L21, W1); W2 = D2^-1 as in 3;
+//   6. W21 = -W2 T1, so W = L11^-1 is complete;
+//   7. all waves store the rest of L11 and W with coalesced row stores.
+// The two 64x64 factorisations (~15k cycles each, s_memtime probe) are the
+// serial part; staging, inverses and the MFMA steps add about as much again.
 //
 // Reference counterpart: e62_diag128 (submission.py:4694-4997), which
 // factors a 128x128 block and builds its inverse in one CTA.
@@ -235,31 +236,59 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   }
   __syncthreads();
 
-  // ---- 5. D2 and W2 ----
+  // Final stores of one row chunk: W row r (all 128 columns) and the
+  // lower-triangle chunk of L row r (rows < nb, columns < nb; the factor is
+  // final, so streaming stores).
+  float* Wb = winv + b * (NB * NB);
+  auto store_w = [&](int r, int cc) {
+    gst4(&Wb[r * NB + cc], *reinterpret_cast<const float4*>(&Ws[r * P + cc]));
+  };
+  auto store_l = [&](int r, int cc) {
+    if (r >= nb || cc > r) return;
+    const float4 v = *reinterpret_cast<const float4*>(&Ls[r * P + cc]);
+    float* dst = Lb + (int64_t)r * M.ld + cc;
+    if (vec && cc + 3 < nb) {
+      gst4_nt(dst, v);
+    } else {
+      gst_nt(dst, v.x);
+      if (cc + 1 < nb) gst_nt(dst + 1, v.y);
+      if (cc + 2 < nb) gst_nt(dst + 2, v.z);
+      if (cc + 3 < nb) gst_nt(dst + 3, v.w);
+    }
+  };
+
+  // ---- 5. wavefront 0 factors D2; meanwhile waves 1-3 form T1 = L21 W1
+  //         in the unused upper-right 64x64 block of Ls (never stored) and
+  //         store what is final: W rows 0..63 (W1, zeros to its right), L
+  //         rows 0..63 and L21. ----
   if (wave == 0) {
     const int bad2 = factor64<16, P>(Ls, H, lane);
     if (bad == 0 && bad2 != 0) bad = H + bad2;
-  }
-  __syncthreads();
-  inverse64(Ls, Ws, Ts, H, wave, lane, li, lg);
-
-  // ---- 6. W21 = -W2 (L21 W1); T1 = L21 W1 staged in the unused upper-right
-  //         64x64 block of Ls (zeros above the diagonal, never stored) ----
-  {
-    floatx4 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  } else {
+    for (int t = wave - 1; t < 16; t += 3) {
+      const int ti = t >> 2, tj = t & 3;
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int kk = 0; kk < H; kk += 16)
-        acc[t] = mma_nn(&Ls[(H + 16 * wave) * P + kk], P, &Ws[kk * P + 16 * t], P, acc[t], li, lg);
+        acc = mma_nn(&Ls[(H + 16 * ti) * P + kk], P, &Ws[kk * P + 16 * tj], P, acc, li, lg);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ls[(16 * ti + 4 * lg + v) * P + H + 16 * tj + li] = acc[v];
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Ls[(16 * wave + 4 * lg + v) * P + H + 16 * t + li] = acc[t][v];
+    const int t2 = tid - 64;  // 0..191
+    for (int e = t2; e < H * (NB / 4); e += 192) {  // rows 0..63, 32 chunks each
+      const int r = e >> 5, cc = (e & 31) * 4;
+      store_w(r, cc);
+      if (cc < H) store_l(r, cc);
+    }
+    for (int e = t2; e < H * (H / 4); e += 192) {  // rows 64..127, columns 0..63
+      const int r = H + (e >> 4), cc = (e & 15) * 4;
+      store_l(r, cc);
+    }
   }
   __syncthreads();
+  inverse64(Ls, Ws, Ts, H, wave, lane, li, lg);  // W2
+
+  // ---- 6. W21 = -W2 T1, so W = L11^-1 is complete ----
   {
     floatx4 acc[4];
 #pragma unroll
@@ -277,25 +306,11 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   if (p.info && tid == 0 && bad != 0 && bad <= nb && p.info[M.slot] == 0) p.info[M.slot] = k0 + bad;
   __syncthreads();
 
-  // ---- 7. store L11 (rows < nb, columns < nb, lower chunks) and W ----
-  float* Wb = winv + b * (NB * NB);
-#pragma unroll 4
-  for (int u = 0; u < NB / 8; ++u) {
-    const int r = (tid >> 5) + 8 * u;
-    gst4(&Wb[r * NB + c4], *reinterpret_cast<const float4*>(&Ws[r * P + c4]));
-    if (r < nb && c4 <= r) {
-      const float4 v = *reinterpret_cast<const float4*>(&Ls[r * P + c4]);
-      float* dst = Lb + (int64_t)r * M.ld + c4;
-      // L11 is final: streaming stores
-      if (vec && c4 + 3 < nb) {
-        gst4_nt(dst, v);
-      } else {
-        gst_nt(dst, v.x);
-        if (c4 + 1 < nb) gst_nt(dst + 1, v.y);
-        if (c4 + 2 < nb) gst_nt(dst + 2, v.z);
-        if (c4 + 3 < nb) gst_nt(dst + 3, v.w);
-      }
-    }
+  // ---- 7. the rest: W rows 64..127, L rows 64..127 from column 64 on ----
+  for (int e = tid; e < H * (NB / 4); e += 256) {
+    const int r = H + (e >> 5), cc = (e & 31) * 4;
+    store_w(r, cc);
+    if (cc >= H) store_l(r, cc);
   }
 }
 
