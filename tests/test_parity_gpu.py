@@ -149,6 +149,25 @@ def test_mixed_against_oracle(oracle_mod, batch, n, case):
     assert (np.triu(Lc, 1) == 0).all()
 
 
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_unaligned_rows_wide_updates(oracle_mod, mode):
+    # n = 1001 (rows not 16-byte aligned) and 96 matrices: the outer-block
+    # updates have >= 256 tiles of 256 x 256 and K = 256, the 8-wave 256-tile
+    # SYRK with scalar row loads.  Checked on 8 sampled matrices.
+    batch, n = 96, 1001
+    a = harness.generate_input(batch, n, 2, 4242, device=DEV)
+    L = cholesky(a, mode=mode)
+    ok, msg = harness.check_implementation(a, L)
+    assert ok, msg
+    idx = np.linspace(0, batch - 1, 8).astype(int)
+    sub = a[idx].cpu().numpy()
+    Lc = L[idx].cpu().numpy()
+    fro, _, _ = oracle_mod.residual(sub, Lc)
+    _say(f"[unaligned-wide] {batch}x{n} {mode}: resid={fro.max():.3e} {msg}")
+    assert fro.max() <= (MIXED_RESID if mode == "mixed" else RESID_C * math.sqrt(n) * EPS32)
+    assert (np.triu(Lc, 1) == 0).all()
+
+
 @pytest.mark.parametrize("spec", [s for s in harness.BENCH_SPECS if s["n"] > 256],
                          ids=lambda s: f"{s['batch']}x{s['n']}")
 def test_mixed_bench_shapes(spec):
