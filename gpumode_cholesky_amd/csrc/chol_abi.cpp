@@ -67,6 +67,7 @@ struct SideStream {
   hipStream_t s2 = nullptr;
   hipEvent_t to_side = nullptr;   // recorded on the caller's stream
   hipEvent_t from_side = nullptr; // recorded on the side stream
+  std::mutex enqueue;  // one call's event record / wait pairs at a time
 };
 static std::mutex g_side_mu;
 static SideStream g_side[64];
@@ -92,38 +93,52 @@ static int side_stream(SideStream*& out) {
   return 0;
 }
 
-// Blocked factorisation with one outer block of look-ahead.  Within outer
-// block [K0, K0+W) the 128-wide steps run on the caller's stream s.  Across
-// outer blocks the update is right-looking: the block's trailing update
-// (K = W) is split: the next outer
-// block's columns (narrow, needed next) stay on s; the columns beyond
-// (bulk) go to the side stream, where they overlap the next block's chain
-// of small launches.  A narrow update first waits for the previous bulk
-// update, which wrote the same columns; the caller's stream joins the side
-// stream at the end.
+// Blocked factorisation with one outer block of look-ahead.  Within outer block
+// [K0, K0+W) the 128-wide steps run on the caller's stream s.  Across outer
+// blocks the update is right-looking: the block's trailing update (K = W)
+// is split: the next outer block's columns (narrow, needed next) stay on s;
+// the columns beyond (bulk) go to the side stream, where they overlap the
+// next block's chain of small launches.  A narrow update first waits for the
+// previous bulk update, which wrote the same columns; the caller's stream
+// joins the side streams at the end.  The event pairs of one call are
+// enqueued under the device's lock, so calls from several host threads do
+// not interleave them (their work on the shared side streams runs in order).
 static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipStream_t s) {
-  int rc = launch_copy_lower(p, s);
-  if (rc) return rc;
-  const int n = p.n;
   SideStream* side = nullptr;
+  int rc = side_stream(side);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(side->enqueue);
+  auto order = [](hipEvent_t ev, hipStream_t from, hipStream_t to) -> int {
+    hipError_t e;
+    if ((e = hipEventRecord(ev, from)) != hipSuccess || (e = hipStreamWaitEvent(to, ev, 0)) != hipSuccess) {
+      set_error("look-ahead: event ordering failed: %s", hipGetErrorString(e));
+      return (int)e;
+    }
+    return 0;
+  };
+  if ((rc = launch_copy_lower(p, s))) return rc;
+  const int n = p.n;
   bool bulk_pending = false;
   for (int K0 = 0; K0 < n; K0 += nbo) {
     const int W = (n - K0) < nbo ? (n - K0) : nbo;
+    const int E = K0 + W;  // end of this outer block
     // Inside the block: left-looking when the strip is tall enough that its
     // updates are throughput work (each column block is brought up to date
     // with the block's earlier columns, K = k0 - K0, just before its
     // diagonal step: the strip is read and written once per column block);
     // right-looking (K = 128 updates of all later columns of the block, one
-    // short launch each) when they are latency work.
+    // short launch each) when they are latency work.  These steps stay on
+    // one stream: a cross-stream event pair costs ~10-20 us of idle time on
+    // this stack, more than the microsecond-scale launches it could overlap.
     const bool left = (int64_t)((n - K0) / kNB) * p.batch >= 256;
-    for (int k0 = K0; k0 < K0 + W; k0 += kNB) {
+    for (int k0 = K0; k0 < E; k0 += kNB) {
       if (left && k0 > K0 && (rc = launch_syrk(p, k0, k0 + kNB, K0, k0 - K0, split, s))) return rc;
       if ((rc = launch_diag_factor(p, k0, winv, s))) return rc;
       if (k0 + kNB >= n) break;
       if ((rc = launch_panel(p, k0, winv, s))) return rc;
-      if (!left && k0 + kNB < K0 + W && (rc = launch_syrk(p, k0 + kNB, K0 + W, k0, kNB, split, s))) return rc;
+      if (!left && k0 + kNB < E && (rc = launch_syrk(p, k0 + kNB, E, k0, kNB, split, s))) return rc;
     }
-    const int c0 = K0 + W;  // first column after this outer block
+    const int c0 = E;  // first column after this outer block
     if (c0 >= n) break;
     const int c1 = (n - c0) < nbo ? n : c0 + nbo;  // end of the next outer block
     if (bulk_pending) {
@@ -132,17 +147,9 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
     }
     if ((rc = launch_syrk(p, c0, c1, K0, W, split, s))) return rc;  // narrow
     if (c1 < n) {
-      if (!side && (rc = side_stream(side))) return rc;
-      hipError_t e;
-      if ((e = hipEventRecord(side->to_side, s)) != hipSuccess || (e = hipStreamWaitEvent(side->s2, side->to_side, 0)) != hipSuccess) {
-        set_error("look-ahead: event ordering failed: %s", hipGetErrorString(e));
-        return (int)e;
-      }
+      if ((rc = order(side->to_side, s, side->s2))) return rc;
       if ((rc = launch_syrk(p, c1, n, K0, W, split, side->s2))) return rc;  // bulk
-      if ((e = hipEventRecord(side->from_side, side->s2)) != hipSuccess) {
-        set_error("look-ahead: event record failed: %s", hipGetErrorString(e));
-        return (int)e;
-      }
+      if (hipEventRecord(side->from_side, side->s2) != hipSuccess) return launch_status("hipEventRecord");
       bulk_pending = true;
     }
   }
