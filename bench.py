@@ -152,6 +152,11 @@ def _time(specs, data, mode, steps, warmup, dev):
     for step in range(steps):
         for spec in specs:
             key = (spec["batch"], spec["n"])
+            # Outside the timed region: let the previous shape's work drain
+            # and the clocks settle (4096x32 follows 1x32768, ~50-100 ms of
+            # full-chip load), then flush the caches.
+            torch.cuda.synchronize()
+            time.sleep(0.005)
             harness.flush_caches(dev)
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)

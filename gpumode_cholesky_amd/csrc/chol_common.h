@@ -185,6 +185,16 @@ __device__ __forceinline__ void decode_tile(int idx, int TI, int& ti, int& tj) {
   ti = t + (idx - (int)((int64_t)t * TI - (int64_t)t * (t - 1) / 2));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Remapping block id -> tile index so that every XCD takes a
+// contiguous run of the column-by-column tile list keeps tiles that share
+// operand blocks (the same column, neighbouring rows) in one L2.  A
+// bijection of [0, T) for bid < T, whatever the dispatch order.
+__device__ __forceinline__ int xcd_tile(int bid, int T) {
+  const int xcd = bid & 7, q8 = T >> 3, r8 = T & 7;
+  return xcd * q8 + (xcd < r8 ? xcd : r8) + (bid >> 3);
+}
+
 __device__ __forceinline__ float readlane_f(float x, int l) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
 }

@@ -474,9 +474,10 @@ __global__ __launch_bounds__(kThreads) void k_syrk_mfma(Batch p, int col0, int c
   if (c1 <= col0) return;
   const int TI = (M.n - col0 + T - 1) / T;
   const int TJ = (c1 - col0 + T - 1) / T;
-  if ((int64_t)blockIdx.x >= (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) return;
+  const int64_t T64 = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
+  if ((int64_t)blockIdx.x >= T64) return;
   int ti, tj;
-  decode_tile(blockIdx.x, TI, ti, tj);
+  decode_tile(xcd_tile(blockIdx.x, (int)T64), TI, ti, tj);
   float* Lb = M.L;
   const int i0 = col0 + T * ti;
   const int j0 = col0 + T * tj;
@@ -506,9 +507,10 @@ __global__ __launch_bounds__(kThreads) void k_syrk_once(Batch p, int col0, int c
   if (c1 <= col0) return;
   const int TI = (M.n - col0 + T - 1) / T;
   const int TJ = (c1 - col0 + T - 1) / T;
-  if ((int64_t)blockIdx.x >= (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) return;
+  const int64_t T64 = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
+  if ((int64_t)blockIdx.x >= T64) return;
   int ti, tj;
-  decode_tile(blockIdx.x, TI, ti, tj);
+  decode_tile(xcd_tile(blockIdx.x, (int)T64), TI, ti, tj);
   float* Lb = M.L;
   const int i0 = col0 + T * ti;
   const int j0 = col0 + T * tj;

@@ -110,14 +110,12 @@ __global__ __launch_bounds__(NT) void k_syrk256(Batch p, int col0, int col1, int
   const int TI = (M.n - col0 + BT - 1) / BT;
   const int TJ = (c1 - col0 + BT - 1) / BT;
   const int T = (int)((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2);
-  // XCD-contiguous remap of the tile index (a bijection of [0, T)); grid.x
-  // covers the largest matrix, so ids at or past this matrix's T exit.
+  // XCD-contiguous tile order (xcd_tile); grid.x covers the largest
+  // matrix, so ids at or past this matrix's T exit.
   const int bid = blockIdx.x;
   if (bid >= T) return;
-  const int xcd = bid & 7, q8 = T >> 3, r8 = T & 7;
-  const int idx = xcd * q8 + (xcd < r8 ? xcd : r8) + (bid >> 3);
   int ti, tj;
-  decode_tile(idx, TI, ti, tj);
+  decode_tile(xcd_tile(bid, T), TI, ti, tj);
   const int i0 = col0 + BT * ti;
   const int j0 = col0 + BT * tj;
   float* L = M.L;
