@@ -34,64 +34,81 @@ __device__ __forceinline__ floatx4_b mma_nt_p(const float* X, const float* Y, fl
 // 2 (PW-1-k) issue slots, where a 64-wide register chain spends an LDS
 // round trip and 63-k row updates.  Returns the 1-based first failing
 // pivot (0 if none).
+// One column panel s of factor64: the register chain on rows c0..d+63 and
+// the panel's trailing update.  Accumulates the 1-based failing pivot in bad.
+template <int PW, int P, int S>
+__device__ __forceinline__ void factor64_step(float* Ls, int d, int lane, int& bad) {
+  const int li = lane & 15, lg = lane >> 4;
+  const int c0 = d + PW * S;
+  float* row = &Ls[(d + lane) * P + c0];
+  float a[PW];
+#pragma unroll
+  for (int j = 0; j < PW; j += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(row + j);
+    a[j] = v.x;
+    a[j + 1] = v.y;
+    a[j + 2] = v.z;
+    a[j + 3] = v.w;
+  }
+  const int b = reg_factor_rl<PW>(a, lane, PW * S);
+  if (bad == 0 && b != 0) bad = PW * S + b;
+  if (lane >= PW * S) {
+#pragma unroll
+    for (int j = 0; j < PW; j += 4)
+      *reinterpret_cast<float4*>(row + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+  }
+  wave_lds_sync();
+  // trailing update: lower 16x16 tiles (ti >= tj) of rows / columns
+  // [PW (S+1), 64); reads only panel columns, writes only later ones
+  constexpr int t0 = PW * (S + 1) / 16;
+  if constexpr (t0 < 4) {
+    floatx4_b acc[10];
+    int nt = 0;
+#pragma unroll
+    for (int ti = t0; ti < 4; ++ti)
+#pragma unroll
+      for (int tj = t0; tj <= ti; ++tj) {
+        floatx4_b c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < PW; kk += 16)
+          c = mma_nt_p<P>(&Ls[(d + 16 * ti) * P + c0 + kk], &Ls[(d + 16 * tj) * P + c0 + kk], c, li, lg);
+        acc[nt++] = c;
+      }
+    nt = 0;
+#pragma unroll
+    for (int ti = t0; ti < 4; ++ti)
+#pragma unroll
+      for (int tj = t0; tj <= ti; ++tj) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
+          if (ti != tj || j <= i) Ls[(d + i) * P + d + j] -= acc[nt][v];
+        }
+        ++nt;
+      }
+    wave_lds_sync();
+  }
+}
+
+// One wavefront factors the 64x64 block of Ls (pitch P) at (d, d), blocked
+// by PW columns: for each column panel [c0, c0+PW), lane = row,
+//   1. reg_factor_rl<PW> on rows c0..d+63 yields the panel's diagonal factor
+//      and, for the rows below it, its part of L (the panel's own
+//      triangular solve), broadcasting pivot columns with v_readlane;
+//   2. the lower 16x16 tiles of the trailing block take the panel's rank-PW
+//      update on v_mfma_f32_16x16x4_f32.
+// A pivot step then costs a few dependent VALU operations plus
+// 2 (PW-1-k) issue slots, where a 64-wide register chain spends an LDS
+// round trip and 63-k row updates.  Returns the 1-based first failing
+// pivot (0 if none).
 template <int PW, int P>
 __device__ __forceinline__ int factor64(float* Ls, int d, int lane) {
-  const int li = lane & 15, lg = lane >> 4;
+  static_assert(PW == 16, "four 16-column panels");
   int bad = 0;
-#pragma unroll
-  for (int s = 0; s < 64 / PW; ++s) {
-    const int c0 = d + PW * s;
-    float* row = &Ls[(d + lane) * P + c0];
-    float a[PW];
-#pragma unroll
-    for (int j = 0; j < PW; j += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(row + j);
-      a[j] = v.x;
-      a[j + 1] = v.y;
-      a[j + 2] = v.z;
-      a[j + 3] = v.w;
-    }
-    const int b = reg_factor_rl<PW>(a, lane, PW * s);
-    if (bad == 0 && b != 0) bad = PW * s + b;
-    if (lane >= PW * s) {
-#pragma unroll
-      for (int j = 0; j < PW; j += 4)
-        *reinterpret_cast<float4*>(row + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
-    }
-    wave_lds_sync();
-    // trailing update: lower 16x16 tiles (ti >= tj) of rows / columns
-    // [PW (s+1), 64); reads only panel columns, writes only later ones
-    const int t0 = PW * (s + 1) / 16;
-    if (t0 < 4) {
-      floatx4_b acc[10];
-      int nt = 0;
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-        for (int tj = 0; tj <= ti; ++tj) {
-          if (tj < t0) continue;
-          floatx4_b c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-          for (int kk = 0; kk < PW; kk += 16)
-            c = mma_nt_p<P>(&Ls[(d + 16 * ti) * P + c0 + kk], &Ls[(d + 16 * tj) * P + c0 + kk], c, li, lg);
-          acc[nt++] = c;
-        }
-      nt = 0;
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-        for (int tj = 0; tj <= ti; ++tj) {
-          if (tj < t0) continue;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
-            if (ti != tj || j <= i) Ls[(d + i) * P + d + j] -= acc[nt][v];
-          }
-          ++nt;
-        }
-      wave_lds_sync();
-    }
-  }
+  factor64_step<PW, P, 0>(Ls, d, lane, bad);
+  factor64_step<PW, P, 1>(Ls, d, lane, bad);
+  factor64_step<PW, P, 2>(Ls, d, lane, bad);
+  factor64_step<PW, P, 3>(Ls, d, lane, bad);
   return bad;
 }
 
