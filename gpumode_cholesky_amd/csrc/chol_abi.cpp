@@ -116,8 +116,10 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
     }
     return 0;
   };
-  if ((rc = launch_copy_lower(p, s))) return rc;
   const int n = p.n;
+  // one 128-block (n <= 128): the diagonal-block kernel reads A and writes
+  // all of L itself
+  if (n > kNB && (rc = launch_copy_lower(p, s))) return rc;
   bool bulk_pending = false;
   for (int K0 = 0; K0 < n; K0 += nbo) {
     const int W = (n - K0) < nbo ? (n - K0) : nbo;

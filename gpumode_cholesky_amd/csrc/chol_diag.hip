@@ -116,6 +116,7 @@ struct __align__(16) DiagLds {
   float Ws[NB * P];
 };
 
+template <bool FROM_A, bool LAST>
 __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv) {
   __shared__ DiagLds sm;
   float(*const Ts)[16 * TP] = sm.Ts;
@@ -131,7 +132,12 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   const int nb = min(NB, M.n - k0);
   if (nb <= 0) return;  // (ragged) this matrix has no block at k0
   float* Lb = M.L + (int64_t)k0 * M.ld + k0;
-  const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0);
+  // FROM_A (the whole matrix is this one block): staged from A, and every
+  // row is stored in full, zeros above the diagonal included, so no copy
+  // of tril(A) precedes this kernel.
+  const float* Sb = FROM_A ? M.A + (int64_t)k0 * M.ld + k0 : Lb;
+  const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0) &&
+                   (!FROM_A || (reinterpret_cast<uintptr_t>(M.A) & 15) == 0);
 
   // ---- 1. stage: thread covers rows tid/32 + 8u, columns 4*(tid%32).
   //         All 16 loads are issued before the first LDS store, so the
@@ -144,7 +150,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 #pragma unroll
     for (int u = 0; u < NB / 8; ++u) {
       const int r = (tid >> 5) + 8 * u;
-      stg[u] = gld4_nt(Lb + (int64_t)r * M.ld + (c4 <= r ? c4 : 0));
+      stg[u] = gld4_nt(Sb + (int64_t)r * M.ld + (c4 <= r ? c4 : 0));
     }
 #pragma unroll
     for (int u = 0; u < NB / 8; ++u) {
@@ -163,7 +169,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (r < nb) {
       if (c4 <= r) {
-        const float* src = Lb + (int64_t)r * M.ld + c4;
+        const float* src = Sb + (int64_t)r * M.ld + c4;
         if (vec && c4 + 3 < nb) {
           v = gld4(src);
         } else {
@@ -244,7 +250,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     gst4(&Wb[r * NB + cc], *reinterpret_cast<const float4*>(&Ws[r * P + cc]));
   };
   auto store_l = [&](int r, int cc) {
-    if (r >= nb || cc > r) return;
+    if (r >= nb || cc >= nb || (!FROM_A && cc > r)) return;
     const float4 v = *reinterpret_cast<const float4*>(&Ls[r * P + cc]);
     float* dst = Lb + (int64_t)r * M.ld + cc;
     if (vec && cc + 3 < nb) {
@@ -260,25 +266,29 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   // ---- 5. wavefront 0 factors D2; meanwhile waves 1-3 form T1 = L21 W1
   //         in the unused upper-right 64x64 block of Ls (never stored) and
   //         store what is final: W rows 0..63 (W1, zeros to its right), L
-  //         rows 0..63 and L21. ----
+  //         rows 0..63 and L21.  LAST (no panel follows this block): no W,
+  //         so neither T1 nor W2 / W21 is formed. ----
+  static_assert(!FROM_A || LAST, "a block staged from A is the whole matrix");
   if (wave == 0) {
     const int bad2 = factor64<16, P>(Ls, H, lane);
     if (bad == 0 && bad2 != 0) bad = H + bad2;
   } else {
-    for (int t = wave - 1; t < 16; t += 3) {
-      const int ti = t >> 2, tj = t & 3;
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (!LAST) {
+      for (int t = wave - 1; t < 16; t += 3) {
+        const int ti = t >> 2, tj = t & 3;
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int kk = 0; kk < H; kk += 16)
-        acc = mma_nn(&Ls[(H + 16 * ti) * P + kk], P, &Ws[kk * P + 16 * tj], P, acc, li, lg);
+        for (int kk = 0; kk < H; kk += 16)
+          acc = mma_nn(&Ls[(H + 16 * ti) * P + kk], P, &Ws[kk * P + 16 * tj], P, acc, li, lg);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) Ls[(16 * ti + 4 * lg + v) * P + H + 16 * tj + li] = acc[v];
+        for (int v = 0; v < 4; ++v) Ls[(16 * ti + 4 * lg + v) * P + H + 16 * tj + li] = acc[v];
+      }
     }
     const int t2 = tid - 64;  // 0..191
     for (int e = t2; e < H * (NB / 4); e += 192) {  // rows 0..63, 32 chunks each
       const int r = e >> 5, cc = (e & 31) * 4;
-      store_w(r, cc);
-      if (cc < H) store_l(r, cc);
+      if constexpr (!LAST) store_w(r, cc);
+      if (cc < H || FROM_A) store_l(r, cc);
     }
     for (int e = t2; e < H * (H / 4); e += 192) {  // rows 64..127, columns 0..63
       const int r = H + (e >> 4), cc = (e & 15) * 4;
@@ -286,10 +296,10 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     }
   }
   __syncthreads();
-  inverse64(Ls, Ws, Ts, H, wave, lane, li, lg);  // W2
+  if constexpr (!LAST) {
+    inverse64(Ls, Ws, Ts, H, wave, lane, li, lg);  // W2
 
-  // ---- 6. W21 = -W2 T1, so W = L11^-1 is complete ----
-  {
+    // ---- 6. W21 = -W2 T1, so W = L11^-1 is complete ----
     floatx4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -302,14 +312,14 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int v = 0; v < 4; ++v) Ws[(H + 16 * wave + 4 * lg + v) * P + 16 * t + li] = -acc[t][v];
+    __syncthreads();
   }
   if (p.info && tid == 0 && bad != 0 && bad <= nb && p.info[M.slot] == 0) p.info[M.slot] = k0 + bad;
-  __syncthreads();
 
   // ---- 7. the rest: W rows 64..127, L rows 64..127 from column 64 on ----
   for (int e = tid; e < H * (NB / 4); e += 256) {
     const int r = H + (e >> 5), cc = (e & 31) * 4;
-    store_w(r, cc);
+    if constexpr (!LAST) store_w(r, cc);
     if (cc >= H) store_l(r, cc);
   }
 }
@@ -317,7 +327,18 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 }  // namespace
 
 int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s) {
-  hipLaunchKernelGGL(k_diag_factor, dim3((unsigned)p.batch), dim3(256), 0, s, p, k0, winv);
+  // The block that reaches column n needs no inverse (no panel follows);
+  // a matrix that is one block is staged from A with no copy before it.
+  // (Ragged batches: every matrix must end at this block for LAST.)
+  const bool last = k0 + kNB >= p.n;
+  const bool whole = last && k0 == 0 && !p.Ap;
+  const dim3 g((unsigned)p.batch), b(256);
+  if (whole)
+    hipLaunchKernelGGL((k_diag_factor<true, true>), g, b, 0, s, p, k0, winv);
+  else if (last)
+    hipLaunchKernelGGL((k_diag_factor<false, true>), g, b, 0, s, p, k0, winv);
+  else
+    hipLaunchKernelGGL((k_diag_factor<false, false>), g, b, 0, s, p, k0, winv);
   return launch_status("k_diag_factor");
 }
 

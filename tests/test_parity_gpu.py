@@ -82,6 +82,21 @@ def test_blocked_families(oracle_mod, n, case, cond):
     _check_against_oracle(oracle_mod, a, f"blocked n={n} {case}")
 
 
+@pytest.mark.parametrize("n", [65, 100, 127, 128, 129, 200, 256])
+def test_blocked_route_forced(oracle_mod, n):
+    # The blocked route where the tuner could pick it for small orders: one
+    # 128-block (n <= 128) is staged from A by the diagonal-block kernel with
+    # no copy before it; n > 128 ends with a block that forms no inverse.
+    from gpumode_cholesky_amd import autotune
+    try:
+        autotune.set_table([[n, n, 1, 1 << 20, 128]])
+        for case, cond in (("dense", 2), ("spectrum", 5)):
+            a = harness.generate_input(3, n, cond, 5000 + n, case, device="cpu")
+            _check_against_oracle(oracle_mod, a, f"blocked-forced n={n} {case}")
+    finally:
+        autotune.load_table()
+
+
 @pytest.mark.parametrize("batch,n", [(4096, 32), (1024, 64), (256, 128), (64, 256), (16, 512), (4, 1024)])
 def test_config_shapes_against_oracle(oracle_mod, batch, n):
     # The benchmark workload generator at its real shape; the oracle checks a
