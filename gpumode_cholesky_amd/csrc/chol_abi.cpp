@@ -117,9 +117,12 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
     return 0;
   };
   const int n = p.n;
-  // one 128-block (n <= 128): the diagonal-block kernel reads A and writes
-  // all of L itself
-  if (n > kNB && (rc = launch_copy_lower(p, s))) return rc;
+  // No copy of tril(A) into L: the kernels that first touch a region read
+  // it from A (pf), and L's upper triangle is zeroed once (one 128-block,
+  // n <= 128: the diagonal-block kernel stores whole rows and needs no zeroing).
+  Batch pf = p;
+  pf.first = true;
+  if (n > kNB && (rc = launch_zero_upper(p, s))) return rc;
   bool bulk_pending = false;
   for (int K0 = 0; K0 < n; K0 += nbo) {
     const int W = (n - K0) < nbo ? (n - K0) : nbo;
@@ -133,12 +136,18 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
     // one stream: a cross-stream event pair costs ~10-20 us of idle time on
     // this stack, more than the microsecond-scale launches it could overlap.
     const bool left = (int64_t)((n - K0) / kNB) * p.batch >= 256;
+    // first touch in the first outer block: its diagonal block and panel at
+    // k0 = 0, the first update of each of its column blocks (right-looking:
+    // the step-0 update of all of them; left-looking: each one's update),
+    // and its narrow and bulk updates
+    const Batch& p0 = K0 == 0 ? pf : p;
     for (int k0 = K0; k0 < E; k0 += kNB) {
-      if (left && k0 > K0 && (rc = launch_syrk(p, k0, k0 + kNB, K0, k0 - K0, split, s))) return rc;
-      if ((rc = launch_diag_factor(p, k0, winv, s))) return rc;
+      const Batch& pk = k0 == 0 ? pf : p;
+      if (left && k0 > K0 && (rc = launch_syrk(p0, k0, k0 + kNB, K0, k0 - K0, split, s))) return rc;
+      if ((rc = launch_diag_factor(pk, k0, winv, s))) return rc;
       if (k0 + kNB >= n) break;
-      if ((rc = launch_panel(p, k0, winv, s))) return rc;
-      if (!left && k0 + kNB < E && (rc = launch_syrk(p, k0 + kNB, E, k0, kNB, split, s))) return rc;
+      if ((rc = launch_panel(pk, k0, winv, s))) return rc;
+      if (!left && k0 + kNB < E && (rc = launch_syrk(pk, k0 + kNB, E, k0, kNB, split, s))) return rc;
     }
     const int c0 = E;  // first column after this outer block
     if (c0 >= n) break;
@@ -147,10 +156,10 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
       if (hipStreamWaitEvent(s, side->from_side, 0) != hipSuccess) return launch_status("hipStreamWaitEvent");
       bulk_pending = false;
     }
-    if ((rc = launch_syrk(p, c0, c1, K0, W, split, s))) return rc;  // narrow
+    if ((rc = launch_syrk(p0, c0, c1, K0, W, split, s))) return rc;  // narrow
     if (c1 < n) {
       if ((rc = order(side->to_side, s, side->s2))) return rc;
-      if ((rc = launch_syrk(p, c1, n, K0, W, split, side->s2))) return rc;  // bulk
+      if ((rc = launch_syrk(p0, c1, n, K0, W, split, side->s2))) return rc;  // bulk
       if (hipEventRecord(side->from_side, side->s2) != hipSuccess) return launch_status("hipEventRecord");
       bulk_pending = true;
     }

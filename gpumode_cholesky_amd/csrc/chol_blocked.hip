@@ -1,16 +1,20 @@
-// chol_blocked.hip — blocked right-looking Cholesky for n > 64.
+// chol_blocked.hip — copy, zero-fill and info kernels of the blocked
+// Cholesky for n > 64 (the driver is blocked_potrf in chol_abi.cpp).
 //
-// The factor is built in place in the output L (the input A is only read by
-// the first copy).  The matrix is walked in 64-wide block columns:
-//   diag  : factor the 64x64 diagonal block in one wavefront's registers and
-//           form its inverse W = L11^-1 (kept in the workspace);
-//   panel : L21 = A21 W^T for every row below the block (a GEMM, K = 64);
+// The dense factor is built in place in the output L.  The matrix is walked
+// in 128-wide block columns:
+//   diag  : factor the 128x128 diagonal block in LDS and form its inverse
+//           W = L11^-1 (kept in the workspace; chol_diag.hip);
+//   panel : L21 = A21 W^T for every row below the block (a GEMM, K = 128);
 //   syrk  : C -= P P^T on the lower trailing tiles.
-// The two products run on the matrix cores (chol_gemm.hip).
-// Inner 64-blocks are grouped into an outer block of `nbo` columns: inner
+// The two products run on the matrix cores (chol_gemm.hip, chol_syrk256.hip).
+// Inner blocks are grouped into an outer block of `nbo` columns: inner
 // steps update only the outer block's columns, and one SYRK with K = nbo then
 // updates the rest of the trailing matrix, so the dominant update streams
-// its C tile once per nbo columns instead of once per 64.
+// its C tile once per nbo columns instead of once per 128.
+// The kernels that first touch a region of L read its old values from A, so
+// a dense call moves no copy of tril(A) into L: k_zero_upper only clears
+// what lies above the diagonal.  (The ragged path still copies first.)
 //
 // Reference counterparts: the 128-wide diag+inverse kernel e62_diag128
 // (submission.py:4694-4997), the panel-times-inverse bmm and trailing
@@ -59,6 +63,32 @@ __global__ __launch_bounds__(256) void k_copy_lower4(Batch p) {
   }
 }
 
+// Zeros right of each row's diagonal chunk: L[i][c..c+3] = 0 for every
+// 4-column chunk c > i.  With the first-touch kernels reading A, this is all
+// of L that the blocked path does not otherwise write (the diagonal-block
+// kernel stores each row's diagonal chunk whole, zeros included).  Streaming
+// stores: 2 n^2 bytes where a copy of tril(A) moves 8 n^2.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_zero_upper(Batch p) {
+  const int n = p.n;
+  const int n4 = (n + 3) / 4;
+  const int64_t b = blockIdx.y;
+  float* dst = p.L + b * p.bs;
+  const int64_t total = (int64_t)n * n4;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n4);
+    const int c = 4 * (int)(e - (int64_t)i * n4);
+    if (c <= i) continue;
+    float* q = as_global(dst + (int64_t)i * p.ld + c);
+    if constexpr (VEC) {
+      gst4_nt(q, make_float4(0.f, 0.f, 0.f, 0.f));
+    } else {
+      for (int j = 0; j < 4 && c + j < n; ++j) gst_nt(q + j, 0.f);
+    }
+  }
+}
+
 __global__ void k_zero_info(int32_t* info, int64_t batch) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < batch) info[i] = 0;
@@ -83,6 +113,19 @@ int launch_copy_lower(const Batch& p, hipStream_t s) {
   else
     hipLaunchKernelGGL(k_copy_lower, dim3((unsigned)gx, (unsigned)p.batch), dim3(256), 0, s, p);
   return launch_status("k_copy_lower");
+}
+
+int launch_zero_upper(const Batch& p, hipStream_t s) {
+  const bool vec = !p.Ap && (reinterpret_cast<uintptr_t>(p.L) & 15) == 0 && (p.ld % 4) == 0 &&
+                   (p.bs % 4) == 0 && (p.n % 4) == 0;
+  const int64_t elems = (int64_t)p.n * ((p.n + 3) / 4);
+  int64_t gx = (elems + 255) / 256;
+  if (gx > 4096) gx = 4096;
+  if (vec)
+    hipLaunchKernelGGL((k_zero_upper<true>), dim3((unsigned)gx, (unsigned)p.batch), dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((k_zero_upper<false>), dim3((unsigned)gx, (unsigned)p.batch), dim3(256), 0, s, p);
+  return launch_status("k_zero_upper");
 }
 
 

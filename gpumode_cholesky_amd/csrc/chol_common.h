@@ -34,6 +34,11 @@ struct Batch {
   float* const* Lp = nullptr;
   const int32_t* ns = nullptr;
   const int32_t* orig = nullptr;
+  // Dense blocked path, first touch of a region: the kernel reads the old
+  // values of its output region (C of a trailing update, A21 of a panel,
+  // the diagonal block) from A instead of L, so no copy of tril(A) into L
+  // precedes the factorisation.
+  bool first = false;
 };
 
 // One matrix of a Batch, dense or ragged.
@@ -248,6 +253,7 @@ inline int launch_status(const char* what) {
 int launch_small(const Batch& p, hipStream_t s);                  // n <= 64
 int launch_lds(const Batch& p, hipStream_t s);                    // n <= 256
 int launch_copy_lower(const Batch& p, hipStream_t s);             // L = tril(A)
+int launch_zero_upper(const Batch& p, hipStream_t s);             // L = 0 right of the diagonal chunks
 int launch_diag_factor(const Batch& p, int k0, float* winv,
                        hipStream_t s);                            // kNB x kNB block
 int launch_panel(const Batch& p, int k0, const float* winv,

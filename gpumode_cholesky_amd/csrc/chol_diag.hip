@@ -116,7 +116,7 @@ struct __align__(16) DiagLds {
   float Ws[NB * P];
 };
 
-template <bool FROM_A, bool LAST>
+template <bool FROM_A, bool LAST, bool FULL>
 __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv) {
   __shared__ DiagLds sm;
   float(*const Ts)[16 * TP] = sm.Ts;
@@ -132,9 +132,9 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   const int nb = min(NB, M.n - k0);
   if (nb <= 0) return;  // (ragged) this matrix has no block at k0
   float* Lb = M.L + (int64_t)k0 * M.ld + k0;
-  // FROM_A (the whole matrix is this one block): staged from A, and every
-  // row is stored in full, zeros above the diagonal included, so no copy
-  // of tril(A) precedes this kernel.
+  // FROM_A: the first block, staged from A (nothing has written L yet).
+  // FULL (the whole matrix is this one block): every row is stored in full,
+  // zeros above the diagonal included, so nothing else writes L.
   const float* Sb = FROM_A ? M.A + (int64_t)k0 * M.ld + k0 : Lb;
   const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0) &&
                    (!FROM_A || (reinterpret_cast<uintptr_t>(M.A) & 15) == 0);
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     gst4(&Wb[r * NB + cc], *reinterpret_cast<const float4*>(&Ws[r * P + cc]));
   };
   auto store_l = [&](int r, int cc) {
-    if (r >= nb || cc >= nb || (!FROM_A && cc > r)) return;
+    if (r >= nb || cc >= nb || (!FULL && cc > r)) return;
     const float4 v = *reinterpret_cast<const float4*>(&Ls[r * P + cc]);
     float* dst = Lb + (int64_t)r * M.ld + cc;
     if (vec && cc + 3 < nb) {
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   //         store what is final: W rows 0..63 (W1, zeros to its right), L
   //         rows 0..63 and L21.  LAST (no panel follows this block): no W,
   //         so neither T1 nor W2 / W21 is formed. ----
-  static_assert(!FROM_A || LAST, "a block staged from A is the whole matrix");
+  static_assert(!FULL || (FROM_A && LAST), "a full block is the whole matrix");
   if (wave == 0) {
     const int bad2 = factor64<16, P>(Ls, H, lane);
     if (bad == 0 && bad2 != 0) bad = H + bad2;
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     for (int e = t2; e < H * (NB / 4); e += 192) {  // rows 0..63, 32 chunks each
       const int r = e >> 5, cc = (e & 31) * 4;
       if constexpr (!LAST) store_w(r, cc);
-      if (cc < H || FROM_A) store_l(r, cc);
+      if (cc < H || FULL) store_l(r, cc);
     }
     for (int e = t2; e < H * (H / 4); e += 192) {  // rows 64..127, columns 0..63
       const int r = H + (e >> 4), cc = (e & 15) * 4;
@@ -327,18 +327,21 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 }  // namespace
 
 int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s) {
-  // The block that reaches column n needs no inverse (no panel follows);
-  // a matrix that is one block is staged from A with no copy before it.
-  // (Ragged batches: every matrix must end at this block for LAST.)
+  // The block that reaches column n needs no inverse (no panel follows).
+  // p.first (dense, k0 = 0): staged from A; a matrix that is one block is
+  // then stored in full.  (Ragged batches: every matrix must end at this
+  // block for LAST; they never take p.first.)
   const bool last = k0 + kNB >= p.n;
-  const bool whole = last && k0 == 0 && !p.Ap;
+  const bool from_a = p.first && !p.Ap;
   const dim3 g((unsigned)p.batch), b(256);
-  if (whole)
-    hipLaunchKernelGGL((k_diag_factor<true, true>), g, b, 0, s, p, k0, winv);
+  if (from_a && last)
+    hipLaunchKernelGGL((k_diag_factor<true, true, true>), g, b, 0, s, p, k0, winv);
+  else if (from_a)
+    hipLaunchKernelGGL((k_diag_factor<true, false, false>), g, b, 0, s, p, k0, winv);
   else if (last)
-    hipLaunchKernelGGL((k_diag_factor<false, true>), g, b, 0, s, p, k0, winv);
+    hipLaunchKernelGGL((k_diag_factor<false, true, false>), g, b, 0, s, p, k0, winv);
   else
-    hipLaunchKernelGGL((k_diag_factor<false, false>), g, b, 0, s, p, k0, winv);
+    hipLaunchKernelGGL((k_diag_factor<false, false, false>), g, b, 0, s, p, k0, winv);
   return launch_status("k_diag_factor");
 }
 
