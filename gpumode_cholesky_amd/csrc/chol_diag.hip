@@ -1,22 +1,22 @@
 // chol_diag.hip — factor one 128x128 diagonal block per matrix and form its
 // inverse: the serial link of the blocked path (one launch per 128 columns).
 //
-// One workgroup of 4 wavefronts per matrix, the block resident in LDS:
-//   1. all waves stage the (already updated) block with coalesced row loads;
-//      rows past n are the identity;
-//   2. wavefront 0 factors the top-left 64x64 block D1 (factor64: 16-column
-//      panels in registers, lane = row, pivots broadcast with v_readlane,
-//      MFMA updates between panels);
-//   3. W1 = D1^-1 by 16x16 blocks: the 4 wavefronts invert the four diagonal
-//      16x16 blocks at once (lane = column, forward substitution), then
-//      block rows 1..3 in turn, W_ij = -W_ii sum_{k=j}^{i-1} L_ik W_kj;
-//   4. L21 = A21 W1^T and D2 -= L21 L21^T on v_mfma_f32_16x16x4_f32;
-//   5. wavefront 0 factors D2 while waves 1-3 form T1 = L21 W1 and store the
-//      parts that are final (L rows 0..63, L21, W1); W2 = D2^-1 as in 3;
-//   6. W21 = -W2 T1, so W = L11^-1 is complete;
-//   7. all waves store the rest of L11 and W with coalesced row stores.
-// The two 64x64 factorisations (~15k cycles each, s_memtime probe) are the
-// serial part; staging, inverses and the MFMA steps add about as much again.
+// One workgroup of 4 wavefronts per matrix, the block resident in LDS.
+// Wavefront 0 runs the pivot chains: the 64x64 blocks D1 (top left) and then
+// D2 (bottom right), each as four 16-column panels (factor64_step: register
+// chain, lane = row, pivots broadcast with v_readlane, MFMA update of the
+// panel's trailing tiles).  Everything else runs on wavefronts 1-3 in the
+// shadow of those chains, one panel behind, with a workgroup barrier after
+// each panel:
+//   * waves 1 and 2: L21 of the finished panel (rows 64..127 of its 16
+//     columns, a triangular solve in registers, lane = row; both waves solve,
+//     each keeps its copy in its own LDS slab) and the panel's rank-16 update
+//     of the later A21 columns and of D2;
+//   * wave 3: the next block row of W1 = D1^-1 (and, during D2, of W2);
+//   * during D2: L21 W1 = T1, W21 = -W2 T1 block row by block row, and the
+//     stores of what is final.
+// What is left after the last D2 panel is one block row of W2, one of W21
+// and the final stores.
 //
 // Reference counterpart: e62_diag128 (submission.py:4694-4997), which
 // factors a 128x128 block and builds its inverse in one CTA.
@@ -48,10 +48,23 @@ __device__ __forceinline__ floatx4 mma_nn(const float* X, int px, const float* Y
   return acc;
 }
 
-// C(16x16) += X Y^T, both X and Y rows read as float4 (pitch P), K = 16.
-__device__ __forceinline__ floatx4 mma_nt(const float* X, const float* Y, floatx4 acc, int li, int lg) {
-  const float4 xa = *reinterpret_cast<const float4*>(X + li * P + 4 * lg);
-  const float4 yb = *reinterpret_cast<const float4*>(Y + li * P + 4 * lg);
+constexpr int XP = 16 + 4;  // pitch of the L21 panel slabs
+
+// LDS of the kernel: the staging tiles first, then the block and its
+// inverse (Ls starts below 64 KiB, so most of its accesses encode their
+// offsets as immediates), then the two L21 panel slabs.
+struct __align__(16) DiagLds {
+  float Ts[4][16 * TP];  // per-wave 16x16 staging
+  float Ls[NB * P];
+  float Ws[NB * P];
+  float Xp[2][H * XP];  // waves 1, 2: their copy of the current L21 panel
+};
+
+// C(16x16) += X Y^T, rows of X (pitch px) and Y (pitch py) read as float4, K = 16.
+__device__ __forceinline__ floatx4 mma_nt2(const float* X, int px, const float* Y, int py, floatx4 acc, int li,
+                                           int lg) {
+  const float4 xa = *reinterpret_cast<const float4*>(X + li * px + 4 * lg);
+  const float4 yb = *reinterpret_cast<const float4*>(Y + li * py + 4 * lg);
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, yb.x, acc, 0, 0, 0);
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, yb.y, acc, 0, 0, 0);
   acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.z, yb.z, acc, 0, 0, 0);
@@ -59,62 +72,230 @@ __device__ __forceinline__ floatx4 mma_nt(const float* X, const float* Y, floatx
   return acc;
 }
 
-// Ws(d..d+64, d..d+64) = inverse of the lower-triangular Ls(d.., d..) block,
-// all 4 wavefronts.  Ends with a workgroup barrier.
-__device__ __forceinline__ void inverse64(const float* Ls, float* Ws, float (*Ts)[16 * TP], int d, int wave,
-                                          int lane, int li, int lg) {
-  {  // four diagonal 16x16 inverses at once, wave w -> block w
-    const int d0 = d + 16 * wave;
+// The lower triangle of the 16x16 block of Ls at (d0, d0), read whole
+// into registers (uniform addresses: LDS broadcasts), so the triangular
+// chains that follow never wait on LDS.
+__device__ __forceinline__ void load_tri16(const float* Ls, int d0, float (&L)[16][16]) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+#pragma unroll
+    for (int q = 0; q <= k; q += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(&Ls[(d0 + k) * P + d0 + q]);
+      L[k][q] = v.x;
+      L[k][q + 1] = v.y;
+      L[k][q + 2] = v.z;
+      L[k][q + 3] = v.w;
+    }
+}
+
+// Block row R of W = inverse of the lower-triangular 64x64 block Ls(d.., d..),
+// one wavefront, block rows 0..R-1 of W already in Ws:
+//   W_Rm = -W_RR Q_m, Q_m = sum_{t=m}^{R-1} L_Rt W_tm   (m < R),
+//   W_RR = L_RR^-1.
+// w_row_sums forms Q_0..Q_{R-1} in T[0..R-1]; w_row_finish forms W_RR
+// (lane = column, column-oriented forward substitution: each solved entry
+// is applied to all later ones at once, so the dependent chain is two
+// operations per column) and the products.
+template <int R>
+__device__ __forceinline__ void w_row_sums(const float* Ls, const float* Ws, float (*T)[16 * TP], int d, int li,
+                                           int lg) {
+  const int d0 = d + 16 * R;
+  floatx4 acc[R > 0 ? R : 1];
+#pragma unroll
+  for (int m = 0; m < R; ++m) acc[m] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < R; ++t)
+#pragma unroll
+    for (int m = 0; m <= t; ++m)
+      acc[m] = mma_nn(&Ls[d0 * P + d + 16 * t], P, &Ws[(d + 16 * t) * P + d + 16 * m], P, acc[m], li, lg);
+#pragma unroll
+  for (int m = 0; m < R; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) T[m][(4 * lg + v) * TP + li] = acc[m][v];
+}
+
+template <int R>
+__device__ __forceinline__ void w_row_finish(const float* Ls, float* Ws, float (*T)[16 * TP], int d, int lane,
+                                             int li, int lg) {
+  const int d0 = d + 16 * R;
+  {
+    float L[16][16];
+    load_tri16(Ls, d0, L);
+    float rinv[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) rinv[i] = __builtin_amdgcn_rcpf(L[i][i]);
     const int c = li;
     float w[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const float* Li = &Ls[(d0 + i) * P + d0];
-      float s = (i == c) ? 1.f : 0.f;
+    for (int i = 0; i < 16; ++i) w[i] = (i == c) ? 1.f : 0.f;
 #pragma unroll
-      for (int q = 0; q < i; q += 4) {
-        const float4 v = *reinterpret_cast<const float4*>(Li + q);
-        s = fmaf(-v.x, w[q], s);
-        if (q + 1 < i) s = fmaf(-v.y, w[q + 1], s);
-        if (q + 2 < i) s = fmaf(-v.z, w[q + 2], s);
-        if (q + 3 < i) s = fmaf(-v.w, w[q + 3], s);
-      }
-      w[i] = s * __builtin_amdgcn_rcpf(Li[i]);
+    for (int q = 0; q < 16; ++q) {
+      w[q] *= rinv[q];
+#pragma unroll
+      for (int i = q + 1; i < 16; ++i) w[i] = fmaf(-L[i][q], w[q], w[i]);
     }
     if (lane < 16) {
 #pragma unroll
       for (int i = 0; i < 16; ++i) Ws[(d0 + i) * P + d0 + c] = w[i];
     }
   }
-  __syncthreads();
+  wave_lds_sync();
+  if constexpr (R > 0) {
+    floatx4 r[R];
 #pragma unroll
-  for (int i = 1; i < 4; ++i) {  // block row i, wave j < i
-    if (wave < i) {
-      const int j = wave;
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int k = j; k < i; ++k)
-        acc = mma_nn(&Ls[(d + 16 * i) * P + d + 16 * k], P, &Ws[(d + 16 * k) * P + d + 16 * j], P, acc, li, lg);
-      float* T = Ts[wave];
+    for (int m = 0; m < R; ++m) r[m] = mma_nn(&Ws[d0 * P + d0], P, T[m], TP, floatx4{0.f, 0.f, 0.f, 0.f}, li, lg);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) T[(4 * lg + v) * TP + li] = acc[v];
-      wave_lds_sync();
-      floatx4 r = {0.f, 0.f, 0.f, 0.f};
-      r = mma_nn(&Ws[(d + 16 * i) * P + d + 16 * i], P, T, TP, r, li, lg);
+    for (int m = 0; m < R; ++m)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) Ws[(d + 16 * i + 4 * lg + v) * P + d + 16 * j + li] = -r[v];
+      for (int v = 0; v < 4; ++v) Ws[(d0 + 4 * lg + v) * P + d + 16 * m + li] = -r[m][v];
+  }
+  wave_lds_sync();
+}
+
+template <int R>
+__device__ __forceinline__ void w_block_row(const float* Ls, float* Ws, float (*T)[16 * TP], int d, int lane, int li,
+                                            int lg) {
+  w_row_sums<R>(Ls, Ws, T, d, li, lg);
+  w_row_finish<R>(Ls, Ws, T, d, lane, li, lg);
+}
+
+// L21 of D1 panel Q (columns 16Q..16Q+15, rows 64..127; lane = row 64 +
+// lane): x = a L_QQ^-T by column-oriented forward substitution, a = the
+// panel's A21 columns (already brought up to date by panels < Q).  The
+// result goes to x and to the wave's slab Xq (row lane, pitch XP).
+template <int Q>
+__device__ __forceinline__ void l21_panel(const float* Ls, float* Xq, int lane, float (&x)[16]) {
+  const float* arow = &Ls[(H + lane) * P + 16 * Q];
+#pragma unroll
+  for (int j = 0; j < 16; j += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(arow + j);
+    x[j] = v.x;
+    x[j + 1] = v.y;
+    x[j + 2] = v.z;
+    x[j + 3] = v.w;
+  }
+  float L[16][16];
+  load_tri16(Ls, 16 * Q, L);
+  float rinv[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) rinv[k] = __builtin_amdgcn_rcpf(L[k][k]);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    x[m] *= rinv[m];
+#pragma unroll
+    for (int k = m + 1; k < 16; ++k) x[k] = fmaf(-x[m], L[k][m], x[k]);
+  }
+#pragma unroll
+  for (int j = 0; j < 16; j += 4)
+    *reinterpret_cast<float4*>(&Xq[lane * XP + j]) = make_float4(x[j], x[j + 1], x[j + 2], x[j + 3]);
+  wave_lds_sync();
+}
+
+// Rank-16 update by L21 panel Q (in Xq) of the later A21 columns
+// (A21[:, 16tj..] -= X L[16tj.., panel Q]^T, tj > Q) and of the lower tiles
+// of D2 (D2 -= X X^T); wave HH (0 or 1) of the two takes every other tile.
+// All products first, then the read-modify-writes.
+template <int Q, int HH>
+__device__ __forceinline__ void l21_updates(float* Ls, const float* Xq, int li, int lg) {
+  constexpr int NA = 4 * (3 - Q);  // A21 tiles
+  constexpr int NT = NA + 10;      // + lower tiles of D2
+  constexpr int NM = (NT - HH + 1) / 2;
+  floatx4 acc[NM];
+  int off[NM];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) {
+    const int t = HH + 2 * k;
+    if (t < NA) {
+      const int ti = t & 3, tj = Q + 1 + (t >> 2);
+      acc[k] = mma_nt2(&Xq[16 * ti * XP], XP, &Ls[16 * tj * P + 16 * Q], P, floatx4{0.f, 0.f, 0.f, 0.f}, li, lg);
+      off[k] = (H + 16 * ti + 4 * lg) * P + 16 * tj + li;
+    } else {
+      const int u = t - NA;  // lower tile u of D2, row by row: (0,0) (1,0) (1,1) (2,0) ...
+      const int ti = u >= 6 ? 3 : u >= 3 ? 2 : u >= 1 ? 1 : 0;
+      const int tj = u - ti * (ti + 1) / 2;
+      acc[k] = mma_nt2(&Xq[16 * ti * XP], XP, &Xq[16 * tj * XP], XP, floatx4{0.f, 0.f, 0.f, 0.f}, li, lg);
+      off[k] = (H + 16 * ti + 4 * lg) * P + H + 16 * tj + li;
+      if (ti == tj) {  // diagonal tile: only j <= i
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (li > 4 * lg + v) acc[k][v] = 0.f;
+      }
     }
-    __syncthreads();
+  }
+  float old[NM][4];
+#pragma unroll
+  for (int k = 0; k < NM; ++k)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) old[k][v] = Ls[off[k] + v * P];
+#pragma unroll
+  for (int k = 0; k < NM; ++k)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Ls[off[k] + v * P] = old[k][v] - acc[k][v];
+}
+
+// T1 = L21 W1 (into the upper-right 64x64 block of Ls, which L never uses),
+// column tiles TA and TB, all four row tiles; W1 is lower triangular, so
+// tile column tj takes K = 16 (4 - tj).
+template <int TA, int TB>
+__device__ __forceinline__ void t1_tiles(float* Ls, const float* Ws, int li, int lg) {
+  floatx4 acc[2][4];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    constexpr int tjs[2] = {TA, TB};
+    const int tj = tjs[u];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) {
+      acc[u][ti] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k >= tj)
+          acc[u][ti] = mma_nn(&Ls[(H + 16 * ti) * P + 16 * k], P, &Ws[16 * k * P + 16 * tj], P, acc[u][ti], li, lg);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    constexpr int tjs[2] = {TA, TB};
+    const int tj = tjs[u];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ls[(16 * ti + 4 * lg + v) * P + H + 16 * tj + li] = acc[u][ti][v];
   }
 }
 
-// LDS of the kernel: the staging tiles first, then the block and its
-// inverse (Ls starts below 64 KiB, so most of its accesses encode their
-// offsets as immediates).
-struct __align__(16) DiagLds {
-  float Ts[4][16 * TP];  // per-wave 16x16 staging
-  float Ls[NB * P];
-  float Ws[NB * P];
-};
+// W21 block row R, column tiles TA and TB: -sum_{m<=R} W2_Rm T1_m,tj.
+template <int R, int TA, int TB>
+__device__ __forceinline__ void w21_tiles(const float* Ls, float* Ws, int li, int lg) {
+  floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+  for (int m = 0; m <= R; ++m) {
+    acc[0] = mma_nn(&Ws[(H + 16 * R) * P + H + 16 * m], P, &Ls[16 * m * P + H + 16 * TA], P, acc[0], li, lg);
+    acc[1] = mma_nn(&Ws[(H + 16 * R) * P + H + 16 * m], P, &Ls[16 * m * P + H + 16 * TB], P, acc[1], li, lg);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    Ws[(H + 16 * R + 4 * lg + v) * P + 16 * TA + li] = -acc[0][v];
+    Ws[(H + 16 * R + 4 * lg + v) * P + 16 * TB + li] = -acc[1][v];
+  }
+}
+
+// U = T1_3,tj - sum_{m<3} Q_m T1_m,tj for column tiles TA, TB (Q_m in T[m]),
+// into W21's block row 3, which then becomes -W2_33 U.
+template <int TA, int TB>
+__device__ __forceinline__ void u_tiles(const float* Ls, float* Ws, const float (*T)[16 * TP], int li, int lg) {
+  floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    acc[0] = mma_nn(T[m], TP, &Ls[16 * m * P + H + 16 * TA], P, acc[0], li, lg);
+    acc[1] = mma_nn(T[m], TP, &Ls[16 * m * P + H + 16 * TB], P, acc[1], li, lg);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int rr = 48 + 4 * lg + v;
+    Ws[(H + rr) * P + 16 * TA + li] = Ls[rr * P + H + 16 * TA + li] - acc[0][v];
+    Ws[(H + rr) * P + 16 * TB + li] = Ls[rr * P + H + 16 * TB + li] - acc[1][v];
+  }
+}
 
 template <bool FROM_A, bool LAST, bool FULL>
 __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv) {
@@ -124,7 +305,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   float* const Ws = sm.Ws;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15;
   const int lg = lane >> 4;
   const int64_t b = blockIdx.x;
@@ -195,132 +376,207 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   for (int u = 0; u < NB / 8; ++u) {
     const int r = (tid >> 5) + 8 * u;
     *reinterpret_cast<float4*>(&Ls[r * P + c4]) = stg[u];
-    *reinterpret_cast<float4*>(&Ws[r * P + c4]) = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (!LAST) *reinterpret_cast<float4*>(&Ws[r * P + c4]) = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __syncthreads();
 
-  // ---- 2-3. D1 and W1 ----
-  int bad = 0;
-  if (wave == 0) bad = factor64<16, P>(Ls, 0, lane);
-  __syncthreads();
-  inverse64(Ls, Ws, Ts, 0, wave, lane, li, lg);
-
-  // ---- 4a. L21 = A21 W1^T: 16 output tiles, 4 per wave, K = 64 ----
-  {
-    floatx4 acc[4];
+  // ---- 2. D1 by wavefront 0; one panel behind it, waves 1-2 form that
+  //         panel's L21 and its rank-16 update of A21 and D2, wave 3 the
+  //         next block row of W1.  Wave 1 moves its L21 panel into Ls one
+  //         round later, when no wave reads those A21 columns any more. ----
+  float* Xq = sm.Xp[wave == 2 ? 1 : 0];
+  int bad1 = 0, bad2 = 0;
+  float x[16];
+  auto put_x = [&](int q) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int ti = wave, tj = t;  // tile row ti, tile column tj (16x16)
-      acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kk = 0; kk < H; kk += 16)
-        acc[t] = mma_nt(&Ls[(H + 16 * ti) * P + kk], &Ws[(16 * tj) * P + kk], acc[t], li, lg);
-    }
-    __syncthreads();  // every read of A21 done before it is overwritten
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Ls[(H + 16 * wave + 4 * lg + v) * P + 16 * t + li] = acc[t][v];
-  }
-  __syncthreads();
-  // ---- 4b. D2 -= L21 L21^T on the 10 lower 16x16 tiles ----
-  for (int t = wave; t < 10; t += 4) {
-    int ti = 0, tj = t;  // t -> (ti >= tj): rows of lengths 1, 2, 3, 4
-    while (tj > ti) {
-      tj -= ti + 1;
-      ++ti;
-    }
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kk = 0; kk < H; kk += 16)
-      acc = mma_nt(&Ls[(H + 16 * ti) * P + kk], &Ls[(H + 16 * tj) * P + kk], acc, li, lg);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
-      if (ti != tj || j <= i) Ls[(H + i) * P + H + j] -= acc[v];
-    }
-  }
-  __syncthreads();
-
-  // Final stores of one row chunk: W row r (all 128 columns) and the
-  // lower-triangle chunk of L row r (rows < nb, columns < nb; the factor is
-  // final, so streaming stores).
-  float* Wb = winv + b * (NB * NB);
-  auto store_w = [&](int r, int cc) {
-    gst4(&Wb[r * NB + cc], *reinterpret_cast<const float4*>(&Ws[r * P + cc]));
+    for (int j = 0; j < 16; j += 4)
+      *reinterpret_cast<float4*>(&Ls[(H + lane) * P + 16 * q + j]) = make_float4(x[j], x[j + 1], x[j + 2], x[j + 3]);
   };
-  auto store_l = [&](int r, int cc) {
-    if (r >= nb || cc >= nb || (!FULL && cc > r)) return;
-    const float4 v = *reinterpret_cast<const float4*>(&Ls[r * P + cc]);
-    float* dst = Lb + (int64_t)r * M.ld + cc;
-    if (vec && cc + 3 < nb) {
-      gst4_nt(dst, v);
-    } else {
-      gst_nt(dst, v.x);
-      if (cc + 1 < nb) gst_nt(dst + 1, v.y);
-      if (cc + 2 < nb) gst_nt(dst + 2, v.z);
-      if (cc + 3 < nb) gst_nt(dst + 3, v.w);
-    }
-  };
-
-  // ---- 5. wavefront 0 factors D2; meanwhile waves 1-3 form T1 = L21 W1
-  //         in the unused upper-right 64x64 block of Ls (never stored) and
-  //         store what is final: W rows 0..63 (W1, zeros to its right), L
-  //         rows 0..63 and L21.  LAST (no panel follows this block): no W,
-  //         so neither T1 nor W2 / W21 is formed. ----
-  static_assert(!FULL || (FROM_A && LAST), "a full block is the whole matrix");
+#define CHOL_L21_ROUND(Q)                            \
+  {                                                  \
+    if (wave == 1 && (Q) > 0) put_x((Q) - 1);        \
+    l21_panel<Q>(Ls, Xq, lane, x);                   \
+    if (wave == 1)                                   \
+      l21_updates<Q, 0>(Ls, Xq, li, lg);             \
+    else                                             \
+      l21_updates<Q, 1>(Ls, Xq, li, lg);             \
+  }
+  if (wave == 0) factor64_step<16, P, 0>(Ls, 0, lane, bad1);
+  __syncthreads();
   if (wave == 0) {
-    const int bad2 = factor64<16, P>(Ls, H, lane);
-    if (bad == 0 && bad2 != 0) bad = H + bad2;
-  } else {
-    if constexpr (!LAST) {
-      for (int t = wave - 1; t < 16; t += 3) {
-        const int ti = t >> 2, tj = t & 3;
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    factor64_step<16, P, 1>(Ls, 0, lane, bad1);
+  } else if (wave < 3) {
+    CHOL_L21_ROUND(0)
+  } else if constexpr (!LAST) {
+    w_block_row<0>(Ls, Ws, Ts, 0, lane, li, lg);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    factor64_step<16, P, 2>(Ls, 0, lane, bad1);
+  } else if (wave < 3) {
+    CHOL_L21_ROUND(1)
+  } else if constexpr (!LAST) {
+    w_block_row<1>(Ls, Ws, Ts, 0, lane, li, lg);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    factor64_step<16, P, 3>(Ls, 0, lane, bad1);
+  } else if (wave < 3) {
+    CHOL_L21_ROUND(2)
+  } else if constexpr (!LAST) {
+    w_block_row<2>(Ls, Ws, Ts, 0, lane, li, lg);
+  }
+  __syncthreads();
+  if (wave == 1 || wave == 2) {
+    CHOL_L21_ROUND(3)
+  } else if (wave == 3) {
+    if constexpr (!LAST) w_block_row<3>(Ls, Ws, Ts, 0, lane, li, lg);
+  }
+#undef CHOL_L21_ROUND
+  __syncthreads();
+
+  // Final stores (one wavefront each call; all LDS reads of a call before
+  // its global stores): L chunks of 32 rows x 64 columns (rows < nb,
+  // columns < nb, lower triangle unless FULL; streaming stores, the factor
+  // is final) and W chunks of 16 rows x 128 columns.
+  float* Wb = winv + b * (NB * NB);
+  auto store_l32 = [&](int r0, int c0) {
+    float4 v[8];
 #pragma unroll
-        for (int kk = 0; kk < H; kk += 16)
-          acc = mma_nn(&Ls[(H + 16 * ti) * P + kk], P, &Ws[kk * P + 16 * tj], P, acc, li, lg);
+    for (int u = 0; u < 8; ++u) {
+      const int e = lane + 64 * u;
+      v[u] = *reinterpret_cast<const float4*>(&Ls[(r0 + (e >> 4)) * P + c0 + (e & 15) * 4]);
+    }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Ls[(16 * ti + 4 * lg + v) * P + H + 16 * tj + li] = acc[v];
+    for (int u = 0; u < 8; ++u) {
+      const int e = lane + 64 * u;
+      const int r = r0 + (e >> 4), cc = c0 + (e & 15) * 4;
+      if (r >= nb || cc >= nb || (!FULL && cc > r)) continue;
+      float* dst = Lb + (int64_t)r * M.ld + cc;
+      if (vec && cc + 3 < nb) {
+        gst4_nt(dst, v[u]);
+      } else {
+        gst_nt(dst, v[u].x);
+        if (cc + 1 < nb) gst_nt(dst + 1, v[u].y);
+        if (cc + 2 < nb) gst_nt(dst + 2, v[u].z);
+        if (cc + 3 < nb) gst_nt(dst + 3, v[u].w);
       }
     }
-    const int t2 = tid - 64;  // 0..191
-    for (int e = t2; e < H * (NB / 4); e += 192) {  // rows 0..63, 32 chunks each
-      const int r = e >> 5, cc = (e & 31) * 4;
-      if constexpr (!LAST) store_w(r, cc);
-      if (cc < H || FULL) store_l(r, cc);
+  };
+  auto store_w16 = [&](int r0) {
+    float4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = lane + 64 * u;
+      v[u] = *reinterpret_cast<const float4*>(&Ws[(r0 + (e >> 5)) * P + (e & 31) * 4]);
     }
-    for (int e = t2; e < H * (H / 4); e += 192) {  // rows 64..127, columns 0..63
-      const int r = H + (e >> 4), cc = (e & 15) * 4;
-      store_l(r, cc);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = lane + 64 * u;
+      gst4(&Wb[(r0 + (e >> 5)) * NB + (e & 31) * 4], v[u]);
+    }
+  };
+  // ---- 3. D2 by wavefront 0, a barrier after each panel (not for LAST:
+  //         no W, so nothing waits on its panels).  Meanwhile: round 0
+  //         stores L rows 0..63, L21 and W rows 0..63; round 1 forms
+  //         T1 = L21 W1 and W2's block row 0; rounds 2-3 W2's next block
+  //         row and W21's block row behind it, round 3 also the sums of
+  //         W2's last block row. ----
+  static_assert(!FULL || (FROM_A && LAST), "a full block is the whole matrix");
+  if (wave == 0) {
+    factor64_step<16, P, 0>(Ls, H, lane, bad2);
+  } else if (wave == 1) {
+    put_x(3);
+    wave_lds_sync();
+    store_l32(H, 0);  // L21
+    store_l32(H + 32, 0);
+  } else if (wave == 2) {
+    store_l32(0, 0);  // L rows 0..63
+    store_l32(32, 0);
+    if constexpr (FULL) {
+      store_l32(0, H);
+      store_l32(32, H);
+    }
+  } else if constexpr (!LAST) {
+#pragma unroll
+    for (int r0 = 0; r0 < H; r0 += 16) store_w16(r0);  // W rows 0..63
+  }
+  if constexpr (!LAST) __syncthreads();
+  if (wave == 0) {
+    factor64_step<16, P, 1>(Ls, H, lane, bad2);
+  } else if constexpr (!LAST) {
+    if (wave < 3) {
+      // T1 column tiles {0, 3} (wave 1) and {1, 2} (wave 2): equal work
+      if (wave == 1)
+        t1_tiles<0, 3>(Ls, Ws, li, lg);
+      else
+        t1_tiles<1, 2>(Ls, Ws, li, lg);
+    } else {
+      w_block_row<0>(Ls, Ws, Ts, H, lane, li, lg);
     }
   }
-  __syncthreads();
+  if constexpr (!LAST) __syncthreads();
+  if (wave == 0) {
+    factor64_step<16, P, 2>(Ls, H, lane, bad2);
+  } else if constexpr (!LAST) {
+    if (wave < 3) {
+      if (wave == 1)
+        w21_tiles<0, 0, 2>(Ls, Ws, li, lg);
+      else
+        w21_tiles<0, 1, 3>(Ls, Ws, li, lg);
+    } else {
+      w_block_row<1>(Ls, Ws, Ts, H, lane, li, lg);
+    }
+  }
+  if constexpr (!LAST) __syncthreads();
+  if (wave == 0) {
+    factor64_step<16, P, 3>(Ls, H, lane, bad2);
+  } else if constexpr (!LAST) {
+    if (wave < 3) {
+      if (wave == 1)
+        w21_tiles<1, 0, 2>(Ls, Ws, li, lg);
+      else
+        w21_tiles<1, 1, 3>(Ls, Ws, li, lg);
+    } else {
+      w_block_row<2>(Ls, Ws, Ts, H, lane, li, lg);
+      w_row_sums<3>(Ls, Ws, Ts, H, li, lg);  // Q_m of W2's block row 3 -> Ts[m]
+    }
+  }
+  if constexpr (!LAST) __syncthreads();
+
+  // ---- 4. the tail: wave 0 stores D2's factor; wave 3 completes W2's
+  //         last block row; waves 1-2 form W21's block row 2 and
+  //         U = T1_3 - sum_{m<3} Q_m T1_m, so that W21's block row 3 is
+  //         -W2_33 U (since W2_3m = -W2_33 Q_m); then that product (one
+  //         tile per wave) and the stores of W rows 64..127. ----
+  if (wave == 0) {
+    const int bad = bad1 != 0 ? bad1 : (bad2 != 0 ? H + bad2 : 0);
+    if (p.info && lane == 0 && bad != 0 && bad <= nb && p.info[M.slot] == 0) p.info[M.slot] = k0 + bad;
+    store_l32(H, H);
+    store_l32(H + 32, H);
+  } else if constexpr (!LAST) {
+    if (wave < 3) {
+      if (wave == 1) {
+        w21_tiles<2, 0, 2>(Ls, Ws, li, lg);
+        u_tiles<0, 2>(Ls, Ws, Ts, li, lg);
+      } else {
+        w21_tiles<2, 1, 3>(Ls, Ws, li, lg);
+        u_tiles<1, 3>(Ls, Ws, Ts, li, lg);
+      }
+    } else {
+      w_row_finish<3>(Ls, Ws, Ts, H, lane, li, lg);
+    }
+  }
   if constexpr (!LAST) {
-    inverse64(Ls, Ws, Ts, H, wave, lane, li, lg);  // W2
-
-    // ---- 6. W21 = -W2 T1, so W = L11^-1 is complete ----
-    floatx4 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kk = 0; kk < H; kk += 16)
-        acc[t] = mma_nn(&Ws[(H + 16 * wave) * P + H + kk], P, &Ls[kk * P + H + 16 * t], P, acc[t], li, lg);
-    }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Ws[(H + 16 * wave + 4 * lg + v) * P + 16 * t + li] = -acc[t][v];
     __syncthreads();
-  }
-  if (p.info && tid == 0 && bad != 0 && bad <= nb && p.info[M.slot] == 0) p.info[M.slot] = k0 + bad;
-
-  // ---- 7. the rest: W rows 64..127, L rows 64..127 from column 64 on ----
-  for (int e = tid; e < H * (NB / 4); e += 256) {
-    const int r = H + (e >> 5), cc = (e & 31) * 4;
-    if constexpr (!LAST) store_w(r, cc);
-    if (cc >= H) store_l(r, cc);
+    {
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mma_nn(&Ws[(H + 48) * P + H + 48], P, &Ws[(H + 48) * P + 16 * wave], P, acc, li, lg);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ws[(H + 48 + 4 * lg + v) * P + 16 * wave + li] = -acc[v];
+    }
+    __syncthreads();
+    store_w16(H + 16 * wave);
   }
 }
 
