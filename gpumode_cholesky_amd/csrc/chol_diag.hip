@@ -114,41 +114,49 @@ __device__ __forceinline__ void w_row_sums(const float* Ls, const float* Ws, flo
     for (int v = 0; v < 4; ++v) T[m][(4 * lg + v) * TP + li] = acc[m][v];
 }
 
+// W_RR = L_RR^-1 for the 16x16 diagonal block at (d0, d0) into Ws.
+__device__ __forceinline__ void w_diag_inv(const float* Ls, float* Ws, int d0, int lane, int li) {
+  float L[16][16];
+  load_tri16(Ls, d0, L);
+  float rinv[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) rinv[i] = __builtin_amdgcn_rcpf(L[i][i]);
+  const int c = li;
+  float w[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) w[i] = (i == c) ? 1.f : 0.f;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    w[q] *= rinv[q];
+#pragma unroll
+    for (int i = q + 1; i < 16; ++i) w[i] = fmaf(-L[i][q], w[q], w[i]);
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Ws[(d0 + i) * P + d0 + c] = w[i];
+  }
+}
+
+// W_Rm = -W_RR T[m] for m in [M0, M1) (W_RR already in Ws).
+template <int M0, int M1>
+__device__ __forceinline__ void w_row_products(float* Ws, const float (*T)[16 * TP], int d, int R, int li, int lg) {
+  const int d0 = d + 16 * R;
+  floatx4 r[M1 - M0];
+#pragma unroll
+  for (int m = M0; m < M1; ++m)
+    r[m - M0] = mma_nn(&Ws[d0 * P + d0], P, T[m], TP, floatx4{0.f, 0.f, 0.f, 0.f}, li, lg);
+#pragma unroll
+  for (int m = M0; m < M1; ++m)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Ws[(d0 + 4 * lg + v) * P + d + 16 * m + li] = -r[m - M0][v];
+}
+
 template <int R>
 __device__ __forceinline__ void w_row_finish(const float* Ls, float* Ws, float (*T)[16 * TP], int d, int lane,
                                              int li, int lg) {
-  const int d0 = d + 16 * R;
-  {
-    float L[16][16];
-    load_tri16(Ls, d0, L);
-    float rinv[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) rinv[i] = __builtin_amdgcn_rcpf(L[i][i]);
-    const int c = li;
-    float w[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) w[i] = (i == c) ? 1.f : 0.f;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      w[q] *= rinv[q];
-#pragma unroll
-      for (int i = q + 1; i < 16; ++i) w[i] = fmaf(-L[i][q], w[q], w[i]);
-    }
-    if (lane < 16) {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) Ws[(d0 + i) * P + d0 + c] = w[i];
-    }
-  }
+  w_diag_inv(Ls, Ws, d + 16 * R, lane, li);
   wave_lds_sync();
-  if constexpr (R > 0) {
-    floatx4 r[R];
-#pragma unroll
-    for (int m = 0; m < R; ++m) r[m] = mma_nn(&Ws[d0 * P + d0], P, T[m], TP, floatx4{0.f, 0.f, 0.f, 0.f}, li, lg);
-#pragma unroll
-    for (int m = 0; m < R; ++m)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Ws[(d0 + 4 * lg + v) * P + d + 16 * m + li] = -r[m][v];
-  }
+  if constexpr (R > 0) w_row_products<0, R>(Ws, T, d, R, li, lg);
   wave_lds_sync();
 }
 
@@ -192,45 +200,62 @@ __device__ __forceinline__ void l21_panel(const float* Ls, float* Xq, int lane, 
 }
 
 // Rank-16 update by L21 panel Q (in Xq) of the later A21 columns
-// (A21[:, 16tj..] -= X L[16tj.., panel Q]^T, tj > Q) and of the lower tiles
-// of D2 (D2 -= X X^T); wave HH (0 or 1) of the two takes every other tile.
-// All products first, then the read-modify-writes.
+// (A21[:, 16tj..] -= X L[16tj.., panel Q]^T, tj > Q: read-modify-write, the
+// next panel's solve reads them) and of the lower tiles of D2 (D2 -= X X^T:
+// accumulated in registers over the four panels, subtracted once after the
+// last).  Wave HH (0 or 1) of the two takes every other tile of each kind.
 template <int Q, int HH>
-__device__ __forceinline__ void l21_updates(float* Ls, const float* Xq, int li, int lg) {
+__device__ __forceinline__ void l21_updates(float* Ls, const float* Xq, floatx4 (&d2)[5], int li, int lg) {
   constexpr int NA = 4 * (3 - Q);  // A21 tiles
-  constexpr int NT = NA + 10;      // + lower tiles of D2
-  constexpr int NM = (NT - HH + 1) / 2;
-  floatx4 acc[NM];
-  int off[NM];
+  constexpr int NM = (NA - HH + 1) / 2;
+  floatx4 acc[NM > 0 ? NM : 1];
+  int off[NM > 0 ? NM : 1];
 #pragma unroll
   for (int k = 0; k < NM; ++k) {
     const int t = HH + 2 * k;
-    if (t < NA) {
-      const int ti = t & 3, tj = Q + 1 + (t >> 2);
-      acc[k] = mma_nt2(&Xq[16 * ti * XP], XP, &Ls[16 * tj * P + 16 * Q], P, floatx4{0.f, 0.f, 0.f, 0.f}, li, lg);
-      off[k] = (H + 16 * ti + 4 * lg) * P + 16 * tj + li;
-    } else {
-      const int u = t - NA;  // lower tile u of D2, row by row: (0,0) (1,0) (1,1) (2,0) ...
+    const int ti = t & 3, tj = Q + 1 + (t >> 2);
+    acc[k] = mma_nt2(&Xq[16 * ti * XP], XP, &Ls[16 * tj * P + 16 * Q], P, floatx4{0.f, 0.f, 0.f, 0.f}, li, lg);
+    off[k] = (H + 16 * ti + 4 * lg) * P + 16 * tj + li;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const int u = HH + 2 * k;  // lower tile u of D2, row by row: (0,0) (1,0) (1,1) (2,0) ...
+    const int ti = u >= 6 ? 3 : u >= 3 ? 2 : u >= 1 ? 1 : 0;
+    const int tj = u - ti * (ti + 1) / 2;
+    d2[k] = mma_nt2(&Xq[16 * ti * XP], XP, &Xq[16 * tj * XP], XP, d2[k], li, lg);
+  }
+  if constexpr (NM > 0) {
+    float old[NM][4];
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) old[k][v] = Ls[off[k] + v * P];
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ls[off[k] + v * P] = old[k][v] - acc[k][v];
+  }
+  if constexpr (Q == 3) {
+    float old[5][4];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int u = HH + 2 * k;
       const int ti = u >= 6 ? 3 : u >= 3 ? 2 : u >= 1 ? 1 : 0;
       const int tj = u - ti * (ti + 1) / 2;
-      acc[k] = mma_nt2(&Xq[16 * ti * XP], XP, &Xq[16 * tj * XP], XP, floatx4{0.f, 0.f, 0.f, 0.f}, li, lg);
-      off[k] = (H + 16 * ti + 4 * lg) * P + H + 16 * tj + li;
-      if (ti == tj) {  // diagonal tile: only j <= i
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          if (li > 4 * lg + v) acc[k][v] = 0.f;
-      }
+      for (int v = 0; v < 4; ++v) old[k][v] = Ls[(H + 16 * ti + 4 * lg + v) * P + H + 16 * tj + li];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int u = HH + 2 * k;
+      const int ti = u >= 6 ? 3 : u >= 3 ? 2 : u >= 1 ? 1 : 0;
+      const int tj = u - ti * (ti + 1) / 2;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (ti != tj || li <= 4 * lg + v)  // diagonal tile: only j <= i
+          Ls[(H + 16 * ti + 4 * lg + v) * P + H + 16 * tj + li] = old[k][v] - d2[k][v];
     }
   }
-  float old[NM][4];
-#pragma unroll
-  for (int k = 0; k < NM; ++k)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) old[k][v] = Ls[off[k] + v * P];
-#pragma unroll
-  for (int k = 0; k < NM; ++k)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Ls[off[k] + v * P] = old[k][v] - acc[k][v];
 }
 
 // T1 = L21 W1 (into the upper-right 64x64 block of Ls, which L never uses),
@@ -263,9 +288,23 @@ __device__ __forceinline__ void t1_tiles(float* Ls, const float* Ws, int li, int
   }
 }
 
-// W21 block row R, column tiles TA and TB: -sum_{m<=R} W2_Rm T1_m,tj.
+// C(16x16) += X Y with Y held in registers in the accumulator layout
+// (lane (li, lg) holds Y[4lg+v][li] in y[v]), X rows read as float4, K = 16.
+__device__ __forceinline__ floatx4 mma_nr(const float* X, int px, const floatx4& y, floatx4 acc, int li, int lg) {
+  const float4 xa = *reinterpret_cast<const float4*>(X + li * px + 4 * lg);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, y[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, y[1], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.z, y[2], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.w, y[3], acc, 0, 0, 0);
+  return acc;
+}
+
+// W21 block row R (R < 3), column tiles TA and TB: -sum_{m<=R} W2_Rm T1_m,tj,
+// then the same tiles of U = T1_3 + sum_{t<3} L22_3t W21_t (kept in
+// registers; U is loaded from T1_3 at R = 0), from which W21's block row 3
+// is -W2_33 U: block row 3 of L22 W21 = -T1.
 template <int R, int TA, int TB>
-__device__ __forceinline__ void w21_tiles(const float* Ls, float* Ws, int li, int lg) {
+__device__ __forceinline__ void w21_tiles(const float* Ls, float* Ws, floatx4 (&U)[2], int li, int lg) {
   floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
   for (int m = 0; m <= R; ++m) {
@@ -273,28 +312,22 @@ __device__ __forceinline__ void w21_tiles(const float* Ls, float* Ws, int li, in
     acc[1] = mma_nn(&Ws[(H + 16 * R) * P + H + 16 * m], P, &Ls[16 * m * P + H + 16 * TB], P, acc[1], li, lg);
   }
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    Ws[(H + 16 * R + 4 * lg + v) * P + 16 * TA + li] = -acc[0][v];
-    Ws[(H + 16 * R + 4 * lg + v) * P + 16 * TB + li] = -acc[1][v];
-  }
-}
-
-// U = T1_3,tj - sum_{m<3} Q_m T1_m,tj for column tiles TA, TB (Q_m in T[m]),
-// into W21's block row 3, which then becomes -W2_33 U.
-template <int TA, int TB>
-__device__ __forceinline__ void u_tiles(const float* Ls, float* Ws, const float (*T)[16 * TP], int li, int lg) {
-  floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    acc[0] = mma_nn(T[m], TP, &Ls[16 * m * P + H + 16 * TA], P, acc[0], li, lg);
-    acc[1] = mma_nn(T[m], TP, &Ls[16 * m * P + H + 16 * TB], P, acc[1], li, lg);
-  }
+  for (int u = 0; u < 2; ++u) acc[u] = -acc[u];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int rr = 48 + 4 * lg + v;
-    Ws[(H + rr) * P + 16 * TA + li] = Ls[rr * P + H + 16 * TA + li] - acc[0][v];
-    Ws[(H + rr) * P + 16 * TB + li] = Ls[rr * P + H + 16 * TB + li] - acc[1][v];
+    Ws[(H + 16 * R + 4 * lg + v) * P + 16 * TA + li] = acc[0][v];
+    Ws[(H + 16 * R + 4 * lg + v) * P + 16 * TB + li] = acc[1][v];
   }
+  if constexpr (R == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      U[0][v] = Ls[(48 + 4 * lg + v) * P + H + 16 * TA + li];
+      U[1][v] = Ls[(48 + 4 * lg + v) * P + H + 16 * TB + li];
+    }
+  }
+  const float* L3 = &Ls[(H + 48) * P + H + 16 * R];  // L22_3R
+  U[0] = mma_nr(L3, P, acc[0], U[0], li, lg);
+  U[1] = mma_nr(L3, P, acc[1], U[1], li, lg);
 }
 
 template <bool FROM_A, bool LAST, bool FULL>
@@ -387,6 +420,9 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   float* Xq = sm.Xp[wave == 2 ? 1 : 0];
   int bad1 = 0, bad2 = 0;
   float x[16];
+  floatx4 d2[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) d2[k] = floatx4{0.f, 0.f, 0.f, 0.f};
   auto put_x = [&](int q) {
 #pragma unroll
     for (int j = 0; j < 16; j += 4)
@@ -397,9 +433,9 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     if (wave == 1 && (Q) > 0) put_x((Q) - 1);        \
     l21_panel<Q>(Ls, Xq, lane, x);                   \
     if (wave == 1)                                   \
-      l21_updates<Q, 0>(Ls, Xq, li, lg);             \
+      l21_updates<Q, 0>(Ls, Xq, d2, li, lg);         \
     else                                             \
-      l21_updates<Q, 1>(Ls, Xq, li, lg);             \
+      l21_updates<Q, 1>(Ls, Xq, d2, li, lg);         \
   }
   if (wave == 0) factor64_step<16, P, 0>(Ls, 0, lane, bad1);
   __syncthreads();
@@ -427,18 +463,16 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     w_block_row<2>(Ls, Ws, Ts, 0, lane, li, lg);
   }
   __syncthreads();
-  if (wave == 1 || wave == 2) {
-    CHOL_L21_ROUND(3)
-  } else if (wave == 3) {
-    if constexpr (!LAST) w_block_row<3>(Ls, Ws, Ts, 0, lane, li, lg);
-  }
+  if (wave == 1 || wave == 2) CHOL_L21_ROUND(3)
 #undef CHOL_L21_ROUND
   __syncthreads();
 
   // Final stores (one wavefront each call; all LDS reads of a call before
   // its global stores): L chunks of 32 rows x 64 columns (rows < nb,
   // columns < nb, lower triangle unless FULL; streaming stores, the factor
-  // is final) and W chunks of 16 rows x 128 columns.
+  // is final) and W chunks of 16 rows x 128 columns.  A whole aligned block
+  // stores every chunk of the 32x64 region: those above the diagonal hold
+  // the zeros that L has there.
   float* Wb = winv + b * (NB * NB);
   auto store_l32 = [&](int r0, int c0) {
     float4 v[8];
@@ -446,6 +480,14 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     for (int u = 0; u < 8; ++u) {
       const int e = lane + 64 * u;
       v[u] = *reinterpret_cast<const float4*>(&Ls[(r0 + (e >> 4)) * P + c0 + (e & 15) * 4]);
+    }
+    if (vec && nb == NB) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = lane + 64 * u;
+        gst4_nt(Lb + (int64_t)(r0 + (e >> 4)) * M.ld + c0 + (e & 15) * 4, v[u]);
+      }
+      return;
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -476,13 +518,15 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
       gst4(&Wb[(r0 + (e >> 5)) * NB + (e & 31) * 4], v[u]);
     }
   };
+
   // ---- 3. D2 by wavefront 0, a barrier after each panel (not for LAST:
   //         no W, so nothing waits on its panels).  Meanwhile: round 0
-  //         stores L rows 0..63, L21 and W rows 0..63; round 1 forms
-  //         T1 = L21 W1 and W2's block row 0; rounds 2-3 W2's next block
-  //         row and W21's block row behind it, round 3 also the sums of
-  //         W2's last block row. ----
+  //         completes W1 and stores L rows 0..63, L21 and W rows 0..63;
+  //         round 1 forms T1 = L21 W1 and W2's block row 0; rounds 2-3
+  //         W2's next block row and W21's block row behind it, with U
+  //         (see w21_tiles) in registers of waves 1-2. ----
   static_assert(!FULL || (FROM_A && LAST), "a full block is the whole matrix");
+  floatx4 U[2];
   if (wave == 0) {
     factor64_step<16, P, 0>(Ls, H, lane, bad2);
   } else if (wave == 1) {
@@ -490,6 +534,10 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     wave_lds_sync();
     store_l32(H, 0);  // L21
     store_l32(H + 32, 0);
+    if constexpr (!LAST) {
+      store_w16(0);
+      store_w16(16);
+    }
   } else if (wave == 2) {
     store_l32(0, 0);  // L rows 0..63
     store_l32(32, 0);
@@ -497,83 +545,80 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
       store_l32(0, H);
       store_l32(32, H);
     }
+    if constexpr (!LAST) store_w16(32);
   } else if constexpr (!LAST) {
-#pragma unroll
-    for (int r0 = 0; r0 < H; r0 += 16) store_w16(r0);  // W rows 0..63
+    w_block_row<3>(Ls, Ws, Ts, 0, lane, li, lg);
+    store_w16(48);
   }
   if constexpr (!LAST) __syncthreads();
   if (wave == 0) {
     factor64_step<16, P, 1>(Ls, H, lane, bad2);
   } else if constexpr (!LAST) {
-    if (wave < 3) {
-      // T1 column tiles {0, 3} (wave 1) and {1, 2} (wave 2): equal work
-      if (wave == 1)
-        t1_tiles<0, 3>(Ls, Ws, li, lg);
-      else
-        t1_tiles<1, 2>(Ls, Ws, li, lg);
-    } else {
+    if (wave == 1)
+      t1_tiles<0, 3>(Ls, Ws, li, lg);  // column tiles {0, 3} and {1, 2}:
+    else if (wave == 2)                // equal work, W1 being lower triangular
+      t1_tiles<1, 2>(Ls, Ws, li, lg);
+    else
       w_block_row<0>(Ls, Ws, Ts, H, lane, li, lg);
-    }
   }
   if constexpr (!LAST) __syncthreads();
   if (wave == 0) {
     factor64_step<16, P, 2>(Ls, H, lane, bad2);
   } else if constexpr (!LAST) {
-    if (wave < 3) {
-      if (wave == 1)
-        w21_tiles<0, 0, 2>(Ls, Ws, li, lg);
-      else
-        w21_tiles<0, 1, 3>(Ls, Ws, li, lg);
-    } else {
+    if (wave == 1)
+      w21_tiles<0, 0, 2>(Ls, Ws, U, li, lg);
+    else if (wave == 2)
+      w21_tiles<0, 1, 3>(Ls, Ws, U, li, lg);
+    else
       w_block_row<1>(Ls, Ws, Ts, H, lane, li, lg);
-    }
   }
   if constexpr (!LAST) __syncthreads();
   if (wave == 0) {
     factor64_step<16, P, 3>(Ls, H, lane, bad2);
   } else if constexpr (!LAST) {
-    if (wave < 3) {
-      if (wave == 1)
-        w21_tiles<1, 0, 2>(Ls, Ws, li, lg);
-      else
-        w21_tiles<1, 1, 3>(Ls, Ws, li, lg);
-    } else {
+    if (wave == 1)
+      w21_tiles<1, 0, 2>(Ls, Ws, U, li, lg);
+    else if (wave == 2)
+      w21_tiles<1, 1, 3>(Ls, Ws, U, li, lg);
+    else
       w_block_row<2>(Ls, Ws, Ts, H, lane, li, lg);
-      w_row_sums<3>(Ls, Ws, Ts, H, li, lg);  // Q_m of W2's block row 3 -> Ts[m]
-    }
   }
   if constexpr (!LAST) __syncthreads();
 
-  // ---- 4. the tail: wave 0 stores D2's factor; wave 3 completes W2's
-  //         last block row; waves 1-2 form W21's block row 2 and
-  //         U = T1_3 - sum_{m<3} Q_m T1_m, so that W21's block row 3 is
-  //         -W2_33 U (since W2_3m = -W2_33 Q_m); then that product (one
-  //         tile per wave) and the stores of W rows 64..127. ----
+  // ---- 4. the tail: wave 0 stores D2's factor and forms the sums Q_m of
+  //         W2's block row 3, wave 3 its diagonal block W2_33, waves 1-2
+  //         W21's block row 2 (and U); then W2_3m = -W2_33 Q_m (waves 0, 3)
+  //         and W21's block row 3 = -W2_33 U (waves 1-2); then the stores
+  //         of W rows 64..127. ----
   if (wave == 0) {
     const int bad = bad1 != 0 ? bad1 : (bad2 != 0 ? H + bad2 : 0);
     if (p.info && lane == 0 && bad != 0 && bad <= nb && p.info[M.slot] == 0) p.info[M.slot] = k0 + bad;
+    if constexpr (!LAST) w_row_sums<3>(Ls, Ws, Ts, H, li, lg);
     store_l32(H, H);
     store_l32(H + 32, H);
   } else if constexpr (!LAST) {
-    if (wave < 3) {
-      if (wave == 1) {
-        w21_tiles<2, 0, 2>(Ls, Ws, li, lg);
-        u_tiles<0, 2>(Ls, Ws, Ts, li, lg);
-      } else {
-        w21_tiles<2, 1, 3>(Ls, Ws, li, lg);
-        u_tiles<1, 3>(Ls, Ws, Ts, li, lg);
-      }
-    } else {
-      w_row_finish<3>(Ls, Ws, Ts, H, lane, li, lg);
-    }
+    if (wave == 1)
+      w21_tiles<2, 0, 2>(Ls, Ws, U, li, lg);
+    else if (wave == 2)
+      w21_tiles<2, 1, 3>(Ls, Ws, U, li, lg);
+    else
+      w_diag_inv(Ls, Ws, H + 48, lane, li);
   }
   if constexpr (!LAST) {
     __syncthreads();
-    {
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = mma_nn(&Ws[(H + 48) * P + H + 48], P, &Ws[(H + 48) * P + 16 * wave], P, acc, li, lg);
+    if (wave == 0) {
+      w_row_products<0, 2>(Ws, Ts, H, 3, li, lg);
+    } else if (wave == 3) {
+      w_row_products<2, 3>(Ws, Ts, H, 3, li, lg);
+    } else {
+      const float* W33 = &Ws[(H + 48) * P + H + 48];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) Ws[(H + 48 + 4 * lg + v) * P + 16 * wave + li] = -acc[v];
+      for (int u = 0; u < 2; ++u) {
+        const int tj = (wave - 1) + 2 * u;
+        const floatx4 r = mma_nr(W33, P, U[u], floatx4{0.f, 0.f, 0.f, 0.f}, li, lg);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ws[(H + 48 + 4 * lg + v) * P + 16 * tj + li] = -r[v];
+      }
     }
     __syncthreads();
     store_w16(H + 16 * wave);
