@@ -118,11 +118,12 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
   };
   const int n = p.n;
   // No copy of tril(A) into L: the kernels that first touch a region read
-  // it from A (pf), and L's upper triangle is zeroed once (one 128-block,
-  // n <= 128: the diagonal-block kernel stores whole rows and needs no zeroing).
+  // it from A (pf).  L's upper triangle is zeroed by extra workgroups of the
+  // first diagonal-block launch where the batch leaves CUs idle, else by
+  // k_zero_upper; each diagonal-block kernel stores its block whole.
   Batch pf = p;
   pf.first = true;
-  if (n > kNB && (rc = launch_zero_upper(p, s))) return rc;
+  if (n > kNB && zero_upper_wgs(p) == 0 && (rc = launch_zero_upper(p, s))) return rc;
   bool bulk_pending = false;
   for (int K0 = 0; K0 < n; K0 += nbo) {
     const int W = (n - K0) < nbo ? (n - K0) : nbo;

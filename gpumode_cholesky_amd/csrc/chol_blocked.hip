@@ -1,4 +1,4 @@
-// chol_blocked.hip — copy, zero-fill and info kernels of the blocked
+// chol_blocked.hip — copy and info kernels of the blocked
 // Cholesky for n > 64 (the driver is blocked_potrf in chol_abi.cpp).
 //
 // The dense factor is built in place in the output L.  The matrix is walked
@@ -13,8 +13,10 @@
 // updates the rest of the trailing matrix, so the dominant update streams
 // its C tile once per nbo columns instead of once per 128.
 // The kernels that first touch a region of L read its old values from A, so
-// a dense call moves no copy of tril(A) into L: k_zero_upper only clears
-// what lies above the diagonal.  (The ragged path still copies first.)
+// a dense call moves no copy of tril(A) into L; the first diagonal-block
+// launch zeros L's upper triangle with extra workgroups on the CUs that the
+// batch leaves idle, or k_zero_upper does when it leaves none.  (The ragged
+// path still copies first.)
 //
 // Reference counterparts: the 128-wide diag+inverse kernel e62_diag128
 // (submission.py:4694-4997), the panel-times-inverse bmm and trailing
@@ -64,10 +66,10 @@ __global__ __launch_bounds__(256) void k_copy_lower4(Batch p) {
 }
 
 // Zeros right of each row's diagonal chunk: L[i][c..c+3] = 0 for every
-// 4-column chunk c > i.  With the first-touch kernels reading A, this is all
-// of L that the blocked path does not otherwise write (the diagonal-block
-// kernel stores each row's diagonal chunk whole, zeros included).  Streaming
-// stores: 2 n^2 bytes where a copy of tril(A) moves 8 n^2.
+// 4-column chunk c > i (the diagonal-block kernel stores each row's
+// diagonal chunk whole).  Used where the batch fills the GPU, so that the
+// first diagonal-block launch has no idle CUs to do it.  Streaming stores:
+// 2 n^2 bytes where a copy of tril(A) moves 8 n^2.
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_zero_upper(Batch p) {
   const int n = p.n;
@@ -115,6 +117,8 @@ int launch_copy_lower(const Batch& p, hipStream_t s) {
   return launch_status("k_copy_lower");
 }
 
+
+
 int launch_zero_upper(const Batch& p, hipStream_t s) {
   const bool vec = !p.Ap && (reinterpret_cast<uintptr_t>(p.L) & 15) == 0 && (p.ld % 4) == 0 &&
                    (p.bs % 4) == 0 && (p.n % 4) == 0;
@@ -127,6 +131,5 @@ int launch_zero_upper(const Batch& p, hipStream_t s) {
     hipLaunchKernelGGL((k_zero_upper<false>), dim3((unsigned)gx, (unsigned)p.batch), dim3(256), 0, s, p);
   return launch_status("k_zero_upper");
 }
-
 
 }  // namespace chol

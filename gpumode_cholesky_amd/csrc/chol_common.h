@@ -254,6 +254,7 @@ int launch_small(const Batch& p, hipStream_t s);                  // n <= 64
 int launch_lds(const Batch& p, hipStream_t s);                    // n <= 256
 int launch_copy_lower(const Batch& p, hipStream_t s);             // L = tril(A)
 int launch_zero_upper(const Batch& p, hipStream_t s);             // L = 0 right of the diagonal chunks
+int zero_upper_wgs(const Batch& p);  // zero-fill workgroups added to the first diagonal-block launch (0: none)
 int launch_diag_factor(const Batch& p, int k0, float* winv,
                        hipStream_t s);                            // kNB x kNB block
 int launch_panel(const Batch& p, int k0, const float* winv,

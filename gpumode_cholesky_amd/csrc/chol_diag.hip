@@ -330,8 +330,39 @@ __device__ __forceinline__ void w21_tiles(const float* Ls, float* Ws, floatx4 (&
   U[1] = mma_nr(L3, P, acc[1], U[1], li, lg);
 }
 
-template <bool FROM_A, bool LAST, bool FULL>
+// L = 0 above the diagonal, in whole 4-column chunks right of each row's
+// diagonal chunk (the diagonal-block kernels store the chunks that hold a
+// diagonal element), by `nwg` workgroups, streaming stores.
+__device__ __forceinline__ void zero_upper(const Batch& p, int wg, int nwg) {
+  const int n = p.n;
+  const int n4 = (n + 3) / 4;
+  const bool vec = ((reinterpret_cast<uintptr_t>(p.L) & 15) == 0) && (p.ld % 4) == 0 && (p.bs % 4) == 0 &&
+                   (n % 4) == 0;
+  const int64_t per = (int64_t)n * n4;
+  const int64_t total = per * p.batch;
+  for (int64_t e = (int64_t)wg * 256 + threadIdx.x; e < total; e += (int64_t)nwg * 256) {
+    const int64_t b = e / per;
+    const int64_t r = e - b * per;
+    const int i = (int)(r / n4);
+    const int c = 4 * (int)(r - (int64_t)i * n4);
+    if (c <= i) continue;
+    float* q = p.L + b * p.bs + (int64_t)i * p.ld + c;
+    if (vec) {
+      gst4_nt(q, make_float4(0.f, 0.f, 0.f, 0.f));
+    } else {
+      for (int j = 0; j < 4 && c + j < n; ++j) gst_nt(q + j, 0.f);
+    }
+  }
+}
+
+template <bool FROM_A, bool LAST>
 __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv) {
+  // Workgroups past the batch (first block of a dense matrix only) zero L's
+  // upper triangle on the CUs that the diagonal blocks leave idle.
+  if (FROM_A && blockIdx.x >= p.batch) {
+    zero_upper(p, blockIdx.x - (int)p.batch, gridDim.x - (int)p.batch);
+    return;
+  }
   __shared__ DiagLds sm;
   float(*const Ts)[16 * TP] = sm.Ts;
   float* const Ls = sm.Ls;
@@ -347,8 +378,6 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   if (nb <= 0) return;  // (ragged) this matrix has no block at k0
   float* Lb = M.L + (int64_t)k0 * M.ld + k0;
   // FROM_A: the first block, staged from A (nothing has written L yet).
-  // FULL (the whole matrix is this one block): every row is stored in full,
-  // zeros above the diagonal included, so nothing else writes L.
   const float* Sb = FROM_A ? M.A + (int64_t)k0 * M.ld + k0 : Lb;
   const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0) &&
                    (!FROM_A || (reinterpret_cast<uintptr_t>(M.A) & 15) == 0);
@@ -469,10 +498,11 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 
   // Final stores (one wavefront each call; all LDS reads of a call before
   // its global stores): L chunks of 32 rows x 64 columns (rows < nb,
-  // columns < nb, lower triangle unless FULL; streaming stores, the factor
-  // is final) and W chunks of 16 rows x 128 columns.  A whole aligned block
-  // stores every chunk of the 32x64 region: those above the diagonal hold
-  // the zeros that L has there.
+  // columns < nb, streaming stores: the factor is final; chunks above the
+  // diagonal hold the zeros that L has there) and W chunks of 16 rows x 128
+  // columns.  The upper-right 64x64 quadrant of the block is stored (zeros)
+  // only when the block is the whole matrix; otherwise the zero fill of L's
+  // upper triangle covers it (every chunk right of a row's diagonal chunk).
   float* Wb = winv + b * (NB * NB);
   auto store_l32 = [&](int r0, int c0) {
     float4 v[8];
@@ -493,7 +523,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     for (int u = 0; u < 8; ++u) {
       const int e = lane + 64 * u;
       const int r = r0 + (e >> 4), cc = c0 + (e & 15) * 4;
-      if (r >= nb || cc >= nb || (!FULL && cc > r)) continue;
+      if (r >= nb || cc >= nb) continue;
       float* dst = Lb + (int64_t)r * M.ld + cc;
       if (vec && cc + 3 < nb) {
         gst4_nt(dst, v[u]);
@@ -525,7 +555,6 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   //         round 1 forms T1 = L21 W1 and W2's block row 0; rounds 2-3
   //         W2's next block row and W21's block row behind it, with U
   //         (see w21_tiles) in registers of waves 1-2. ----
-  static_assert(!FULL || (FROM_A && LAST), "a full block is the whole matrix");
   floatx4 U[2];
   if (wave == 0) {
     factor64_step<16, P, 0>(Ls, H, lane, bad2);
@@ -541,7 +570,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   } else if (wave == 2) {
     store_l32(0, 0);  // L rows 0..63
     store_l32(32, 0);
-    if constexpr (FULL) {
+    if constexpr (FROM_A && LAST) {  // the whole matrix: no zero fill
       store_l32(0, H);
       store_l32(32, H);
     }
@@ -627,22 +656,37 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 
 }  // namespace
 
+// Zero-fill workgroups for the first diagonal-block launch: one per CU the
+// batch leaves idle, and only while each has at most ~160 KiB to zero (one
+// workgroup per CU streams far less than the separate fill kernel, and the
+// fill must end within the diagonal block's time).
+int zero_upper_wgs(const Batch& p) {
+  if (p.batch > 192) return 0;
+  const int wgs = 256 - (int)p.batch;
+  const double bytes = 2.0 * (double)p.n * (double)p.n * (double)p.batch;
+  return bytes <= 160.0 * 1024.0 * wgs ? wgs : 0;
+}
+
 int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s) {
   // The block that reaches column n needs no inverse (no panel follows).
-  // p.first (dense, k0 = 0): staged from A; a matrix that is one block is
-  // then stored in full.  (Ragged batches: every matrix must end at this
-  // block for LAST; they never take p.first.)
+  // p.first (dense, k0 = 0): staged from A.  (Ragged batches: every matrix
+  // must end at this block for LAST; they never take p.first.)
   const bool last = k0 + kNB >= p.n;
   const bool from_a = p.first && !p.Ap;
-  const dim3 g((unsigned)p.batch), b(256);
+  const dim3 b(256);
+  dim3 g((unsigned)p.batch);
+  // The first block of a dense matrix larger than one block also zeros L's
+  // upper triangle, with one extra workgroup per CU that the batch leaves
+  // idle (every workgroup of this kernel takes a CU's LDS).
+  if (from_a && !last) g.x += (unsigned)zero_upper_wgs(p);
   if (from_a && last)
-    hipLaunchKernelGGL((k_diag_factor<true, true, true>), g, b, 0, s, p, k0, winv);
+    hipLaunchKernelGGL((k_diag_factor<true, true>), g, b, 0, s, p, k0, winv);
   else if (from_a)
-    hipLaunchKernelGGL((k_diag_factor<true, false, false>), g, b, 0, s, p, k0, winv);
+    hipLaunchKernelGGL((k_diag_factor<true, false>), g, b, 0, s, p, k0, winv);
   else if (last)
-    hipLaunchKernelGGL((k_diag_factor<false, true, false>), g, b, 0, s, p, k0, winv);
+    hipLaunchKernelGGL((k_diag_factor<false, true>), g, b, 0, s, p, k0, winv);
   else
-    hipLaunchKernelGGL((k_diag_factor<false, false, false>), g, b, 0, s, p, k0, winv);
+    hipLaunchKernelGGL((k_diag_factor<false, false>), g, b, 0, s, p, k0, winv);
   return launch_status("k_diag_factor");
 }
 

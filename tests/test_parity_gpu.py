@@ -97,6 +97,35 @@ def test_blocked_route_forced(oracle_mod, n):
         autotune.load_table()
 
 
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+@pytest.mark.parametrize("n", [32, 64, 100, 128, 200, 256, 383, 512, 1001])
+def test_output_written_whole(n, mode):
+    # The kernels write every element of L, zeros above the diagonal
+    # included (there is no separate zero fill), so an output buffer full of
+    # NaN comes back as the same factor as a fresh one.  n <= 256 also runs
+    # with the blocked route forced.
+    from gpumode_cholesky_amd import autotune
+    a = harness.generate_input(3, n, 3, 7100 + n, "dense", device="cpu").to(DEV)
+    ref = cholesky(a, mode=mode)
+    routes = [None] if n > 256 else [None, [[n, n, 1, 1 << 20, 128]]]
+    try:
+        for table in routes:
+            if table is not None:
+                autotune.set_table(table)
+            fresh = cholesky(a, mode=mode)
+            out = torch.full_like(a, float("nan"))
+            L = cholesky(a, mode=mode, out=out)
+            torch.cuda.synchronize()
+            assert L.data_ptr() == out.data_ptr()
+            assert torch.count_nonzero(torch.triu(L, 1)).item() == 0, f"n={n} route={table}: upper not zero"
+            assert not torch.isnan(L).any().item(), f"n={n} route={table}: element not written"
+            assert torch.equal(L, fresh), f"n={n} route={table}: differs from a fresh output"
+            if table is None:
+                assert torch.equal(L, ref)
+    finally:
+        autotune.load_table()
+
+
 @pytest.mark.parametrize("batch,n", [(4096, 32), (1024, 64), (256, 128), (64, 256), (16, 512), (4, 1024)])
 def test_config_shapes_against_oracle(oracle_mod, batch, n):
     # The benchmark workload generator at its real shape; the oracle checks a
