@@ -142,13 +142,30 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
     // the step-0 update of all of them; left-looking: each one's update),
     // and its narrow and bulk updates
     const Batch& p0 = K0 == 0 ? pf : p;
+    DiagBulk bulk{};
+    const DiagBulk* carry = nullptr;
     for (int k0 = K0; k0 < E; k0 += kNB) {
       const Batch& pk = k0 == 0 ? pf : p;
       if (left && k0 > K0 && (rc = launch_syrk(p0, k0, k0 + kNB, K0, k0 - K0, split, s))) return rc;
-      if ((rc = launch_diag_factor(pk, k0, winv, s))) return rc;
+      if ((rc = launch_diag_factor(pk, k0, winv, s, carry))) return rc;
+      carry = nullptr;
       if (k0 + kNB >= n) break;
       if ((rc = launch_panel(pk, k0, winv, s))) return rc;
-      if (!left && k0 + kNB < E && (rc = launch_syrk(pk, k0 + kNB, E, k0, kNB, split, s))) return rc;
+      if (!left && k0 + kNB < E) {
+        // Right-looking update by this panel: the next column block now (the
+        // next diagonal step needs it); the columns beyond it with the next
+        // diagonal-block launch, on the CUs its blocks leave idle, when they
+        // fit there in about three rounds.
+        const int cn = k0 + 2 * kNB;
+        bulk = DiagBulk{cn, E, k0, pk.first, split};
+        if (cn < E && !p.Ap && p.batch <= 192 && vec_ok(pk) &&
+            diag_bulk_tiles(p, bulk) <= 3 * (256 - p.batch)) {
+          if ((rc = launch_syrk(pk, k0 + kNB, cn, k0, kNB, split, s))) return rc;
+          carry = &bulk;
+        } else if ((rc = launch_syrk(pk, k0 + kNB, E, k0, kNB, split, s))) {
+          return rc;
+        }
+      }
     }
     const int c0 = E;  // first column after this outer block
     if (c0 >= n) break;

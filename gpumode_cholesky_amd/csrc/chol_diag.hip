@@ -22,6 +22,7 @@
 // factors a 128x128 block and builds its inverse in one CTA.
 #include "chol_common.h"
 #include "chol_block.h"
+#include "chol_once.h"
 
 namespace chol {
 namespace {
@@ -355,15 +356,26 @@ __device__ __forceinline__ void zero_upper(const Batch& p, int wg, int nwg) {
   }
 }
 
-template <bool FROM_A, bool LAST>
-__global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv) {
-  // Workgroups past the batch (first block of a dense matrix only) zero L's
-  // upper triangle on the CUs that the diagonal blocks leave idle.
-  if (FROM_A && blockIdx.x >= p.batch) {
-    zero_upper(p, blockIdx.x - (int)p.batch, gridDim.x - (int)p.batch);
+// BULK: 0 none, 1 FP32, 2 BF16x3 SYRK tiles (64x64, one-shot) in the
+// workgroups past the batch (bt tiles per matrix).
+template <bool FROM_A, bool LAST, int BULK>
+__global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv, int bc0, int bc1,
+                                                     int bk0, int bt, int bfirst) {
+  __shared__ DiagLds sm;
+  // Workgroups past the batch run on the CUs that the diagonal blocks leave
+  // idle: the first block of a dense matrix zeros L's upper triangle; a
+  // later one runs the previous step's trailing-update tiles beyond the
+  // next column block (which no diagonal block of this launch reads).
+  if (blockIdx.x >= p.batch) {
+    const int id = blockIdx.x - (int)p.batch;
+    if constexpr (FROM_A) {
+      zero_upper(p, id, gridDim.x - (int)p.batch);
+    } else if constexpr (BULK != 0) {
+      syrk_once_tile<64, true, BULK == 2>(p, id % bt, id / bt, bc0, bc1, bk0, bfirst != 0,
+                                          reinterpret_cast<float*>(&sm));
+    }
     return;
   }
-  __shared__ DiagLds sm;
   float(*const Ts)[16 * TP] = sm.Ts;
   float* const Ls = sm.Ls;
   float* const Ws = sm.Ws;
@@ -667,7 +679,13 @@ int zero_upper_wgs(const Batch& p) {
   return bytes <= 160.0 * 1024.0 * wgs ? wgs : 0;
 }
 
-int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s) {
+int64_t diag_bulk_tiles(const Batch& p, const DiagBulk& u) {
+  const int TI = (p.n - u.c0 + 63) / 64;
+  const int TJ = (u.c1 - u.c0 + 63) / 64;
+  return ((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) * p.batch;
+}
+
+int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const DiagBulk* bulk) {
   // The block that reaches column n needs no inverse (no panel follows).
   // p.first (dense, k0 = 0): staged from A.  (Ragged batches: every matrix
   // must end at this block for LAST; they never take p.first.)
@@ -679,14 +697,42 @@ int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s) {
   // upper triangle, with one extra workgroup per CU that the batch leaves
   // idle (every workgroup of this kernel takes a CU's LDS).
   if (from_a && !last) g.x += (unsigned)zero_upper_wgs(p);
-  if (from_a && last)
-    hipLaunchKernelGGL((k_diag_factor<true, true>), g, b, 0, s, p, k0, winv);
-  else if (from_a)
-    hipLaunchKernelGGL((k_diag_factor<true, false>), g, b, 0, s, p, k0, winv);
-  else if (last)
-    hipLaunchKernelGGL((k_diag_factor<false, true>), g, b, 0, s, p, k0, winv);
-  else
-    hipLaunchKernelGGL((k_diag_factor<false, false>), g, b, 0, s, p, k0, winv);
+  int bc0 = 0, bc1 = 0, bk0 = 0, bt = 1, bfirst = 0;
+  if (bulk) {  // dense, float4-aligned (vec_ok), not the first block
+    const int TI = (p.n - bulk->c0 + 63) / 64;
+    const int TJ = (bulk->c1 - bulk->c0 + 63) / 64;
+    bt = (int)((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2);
+    bc0 = bulk->c0;
+    bc1 = bulk->c1;
+    bk0 = bulk->k0;
+    bfirst = bulk->first ? 1 : 0;
+    g.x += (unsigned)(bt * p.batch);
+  }
+#define CHOL_DIAG_LAUNCH(FA, LA, BU) \
+  hipLaunchKernelGGL((k_diag_factor<FA, LA, BU>), g, b, 0, s, p, k0, winv, bc0, bc1, bk0, bt, bfirst)
+  const int bu = bulk ? (bulk->split ? 2 : 1) : 0;
+  if (from_a && last) {
+    CHOL_DIAG_LAUNCH(true, true, 0);
+  } else if (from_a) {
+    CHOL_DIAG_LAUNCH(true, false, 0);
+  } else if (last) {
+    if (bu == 2) {
+      CHOL_DIAG_LAUNCH(false, true, 2);
+    } else if (bu == 1) {
+      CHOL_DIAG_LAUNCH(false, true, 1);
+    } else {
+      CHOL_DIAG_LAUNCH(false, true, 0);
+    }
+  } else {
+    if (bu == 2) {
+      CHOL_DIAG_LAUNCH(false, false, 2);
+    } else if (bu == 1) {
+      CHOL_DIAG_LAUNCH(false, false, 1);
+    } else {
+      CHOL_DIAG_LAUNCH(false, false, 0);
+    }
+  }
+#undef CHOL_DIAG_LAUNCH
   return launch_status("k_diag_factor");
 }
 

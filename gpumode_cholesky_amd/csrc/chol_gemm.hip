@@ -20,55 +20,10 @@
 // (submission.py:5163-5184, :2529-2534, :3362-3401), which ran on TF32/FP16
 // tensor cores; here all-FP32 matrix cores.
 #include "chol_common.h"
+#include "chol_once.h"
 
 namespace chol {
 namespace {
-
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-
-constexpr int KC = 32;        // K chunk staged in LDS
-constexpr int KP = KC + 4;    // LDS row pitch (floats)
-constexpr int kThreads = 256;
-
-template <int ROWS, bool VEC, int KW = KC>
-__device__ __forceinline__ void load_rows(float4 (&reg)[ROWS * KW / 4 / kThreads], const float* base,
-                                          int64_t ld, int row0, int rlim, int k, int tid) {
-  // ROWS x KW floats as float4 chunks; thread t takes chunks t, t+256, ...
-  constexpr int PER = ROWS * KW / 4 / kThreads;
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int c = tid + u * kThreads;
-    const int r = c / (KW / 4);
-    const int q = (c % (KW / 4)) * 4;
-    const int gr = row0 + r;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (gr < rlim) {
-      const float* src = base + (int64_t)gr * ld + k + q;
-      if constexpr (VEC) {
-        v = gld4(src);
-      } else {
-        v.x = gld(src);
-        v.y = gld(src + 1);
-        v.z = gld(src + 2);
-        v.w = gld(src + 3);
-      }
-    }
-    reg[u] = v;
-  }
-}
-
-template <int ROWS, int KW = KC, int PITCH = KP>
-__device__ __forceinline__ void store_rows(float* __restrict__ lds, const float4 (&reg)[ROWS * KW / 4 / kThreads],
-                                           int tid) {
-  constexpr int PER = ROWS * KW / 4 / kThreads;
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int c = tid + u * kThreads;
-    const int r = c / (KW / 4);
-    const int q = (c % (KW / 4)) * 4;
-    *reinterpret_cast<float4*>(lds + r * PITCH + q) = reg[u];
-  }
-}
 
 // Write back a wave's FM x FN blocks of 16x16 accumulators (lane holds
 // D[4*lg + v][li] of each block) at rows r0.., columns c0..:
@@ -211,32 +166,6 @@ __device__ __forceinline__ void gemm_tile(const float* Abase, int64_t lda, int a
 // The dropped loA loB^T term and the split remainder are ~2^-16 of |a b|.
 // Operand layout of 16x16x32 bf16: A[i][k] for k = 8g..8g+7 in lane i + 16g,
 // B likewise by column; D as for 16x16x4 f32.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-constexpr int KPH = KC + 8;  // bf16 LDS row pitch (80 bytes)
-
-template <int ROWS, int KW = KC, int PITCH = KPH>
-__device__ __forceinline__ void store_rows_split(__bf16* __restrict__ hi, __bf16* __restrict__ lo,
-                                                 const float4 (&reg)[ROWS * KW / 4 / kThreads], int tid) {
-  constexpr int PER = ROWS * KW / 4 / kThreads;
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int c = tid + u * kThreads;
-    const int r = c / (KW / 4);
-    const int q = (c % (KW / 4)) * 4;
-    const float4 v = reg[u];
-    const __bf16 h0 = (__bf16)v.x, h1 = (__bf16)v.y, h2 = (__bf16)v.z, h3 = (__bf16)v.w;
-    const __bf16 l0 = (__bf16)(v.x - (float)h0), l1 = (__bf16)(v.y - (float)h1);
-    const __bf16 l2 = (__bf16)(v.z - (float)h2), l3 = (__bf16)(v.w - (float)h3);
-    bf16x2* ph = reinterpret_cast<bf16x2*>(hi + r * PITCH + q);
-    bf16x2* pl = reinterpret_cast<bf16x2*>(lo + r * PITCH + q);
-    ph[0] = bf16x2{h0, h1};
-    ph[1] = bf16x2{h2, h3};
-    pl[0] = bf16x2{l0, l1};
-    pl[1] = bf16x2{l2, l3};
-  }
-}
-
 // C -= A B^T on the lower part (i >= j), j < jlim, i < ilim, BF16x3 products.
 template <int BM, int BN, bool VEC>
 __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda, int a_row0, int a_lim, int ka,
@@ -312,158 +241,6 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda
   epilogue<FM, FN, 0>(acc, Cbase, Csrc, ldc, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
 }
 
-// ------------------------------------------------------ one-shot K = 128 ----
-// Latency form of the per-step launches (K = 128, too few tiles to fill the
-// GPU, as for one or a few large matrices): a workgroup requests all of its
-// A and B rows (64 KiB for a 64x64 tile) and its C tile at once, stages A
-// and B into LDS in one pass and runs the whole K extent without further
-// barriers.  A chunked pipeline pays one global-memory round trip per
-// 32-wide chunk plus one for C, and with a few short tiles per CU those
-// round trips, not the matrix cores, set the time.
-constexpr int K1 = kNB;       // K extent of the one-shot form
-constexpr int K1P = K1 + 4;   // FP32 LDS row pitch (floats)
-constexpr int K1H = K1 + 8;   // BF16 LDS row pitch (elements)
-
-// MODE 0: C -= A B^T on the lower part (i >= j), i < ilim, j < jlim.
-// MODE 1: C  = A B^T, i < ilim (panel; A and C alias row-for-row).
-// SPLIT: BF16x3 products as in gemm_tile_bf16x3 (MODE 0 only).
-template <int BM, int BN, int MODE, bool VEC, bool SPLIT>
-__device__ __forceinline__ void gemm_tile_once(const float* Abase, int64_t lda, int a_row0, int a_lim, int ka,
-                                               const float* Bbase, int64_t ldb, int b_row0, int b_lim, int kb,
-                                               float* Cbase, const float* Csrc, int64_t ldc, int c_row0,
-                                               int c_col0, int c_ilim, int c_jlim) {
-  static_assert(!SPLIT || MODE == 0, "split products are used for the trailing update only");
-  constexpr int WM = BM / 2, WN = BN / 2;
-  constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int PA = BM * K1 / 4 / kThreads;
-  constexpr int PB = BN * K1 / 4 / kThreads;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = (wave >> 1) * WM;
-  const int wn = (wave & 1) * WN;
-  const int li = lane & 15;
-  const int lg = lane >> 4;
-
-  float4 ra[PA], rb[PB];
-  load_rows<BM, VEC, K1>(ra, Abase, lda, a_row0, a_lim, ka, tid);
-  load_rows<BN, VEC, K1>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
-
-  // C tile of this wave: offsets, masks and (MODE 0) the old values, whose
-  // loads fly together with the operand loads.  A masked element reads
-  // C[0][0], always valid; only the stores are masked.
-  const int r0 = c_row0 + wm, cq = c_col0 + wn;
-  int64_t off[FM][FN][4];
-  bool ok[FM][FN][4];
-  float old[FM][FN][4];
-#pragma unroll
-  for (int a = 0; a < FM; ++a)
-#pragma unroll
-    for (int b = 0; b < FN; ++b)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = r0 + a * 16 + 4 * lg + v;
-        const int j = cq + b * 16 + li;
-        const bool m = (MODE == 0) ? (i < c_ilim && j < c_jlim && j <= i) : (i < c_ilim);
-        ok[a][b][v] = m;
-        off[a][b][v] = m ? (int64_t)i * ldc + j : 0;
-        if constexpr (MODE == 0) old[a][b][v] = gld(Csrc + off[a][b][v]);
-      }
-
-  floatx4 acc[FM][FN];
-#pragma unroll
-  for (int a = 0; a < FM; ++a)
-#pragma unroll
-    for (int b = 0; b < FN; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-  if constexpr (SPLIT) {
-    __shared__ __align__(16) __bf16 Ah[BM * K1H];
-    __shared__ __align__(16) __bf16 Al[BM * K1H];
-    __shared__ __align__(16) __bf16 Bh[BN * K1H];
-    __shared__ __align__(16) __bf16 Bl[BN * K1H];
-    store_rows_split<BM, K1, K1H>(Ah, Al, ra, tid);
-    store_rows_split<BN, K1, K1H>(Bh, Bl, rb, tid);
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < K1; ks += 32) {
-      bf16x8 fah[FM], fal[FM], fbh[FN], fbl[FN];
-#pragma unroll
-      for (int a = 0; a < FM; ++a) {
-        const int o = (wm + a * 16 + li) * K1H + ks + 8 * lg;
-        fah[a] = *reinterpret_cast<const bf16x8*>(&Ah[o]);
-        fal[a] = *reinterpret_cast<const bf16x8*>(&Al[o]);
-      }
-#pragma unroll
-      for (int b = 0; b < FN; ++b) {
-        const int o = (wn + b * 16 + li) * K1H + ks + 8 * lg;
-        fbh[b] = *reinterpret_cast<const bf16x8*>(&Bh[o]);
-        fbl[b] = *reinterpret_cast<const bf16x8*>(&Bl[o]);
-      }
-#pragma unroll
-      for (int a = 0; a < FM; ++a)
-#pragma unroll
-        for (int b = 0; b < FN; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fal[a], fbh[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbl[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbh[b], acc[a][b], 0, 0, 0);
-        }
-    }
-  } else {
-    __shared__ __align__(16) float As[BM * K1P];
-    __shared__ __align__(16) float Bs[BN * K1P];
-    store_rows<BM, K1, K1P>(As, ra, tid);
-    store_rows<BN, K1, K1P>(Bs, rb, tid);
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < K1; ks += 16) {
-      float4 fa[FM], fb[FN];
-#pragma unroll
-      for (int a = 0; a < FM; ++a)
-        fa[a] = *reinterpret_cast<const float4*>(As + (wm + a * 16 + li) * K1P + ks + 4 * lg);
-#pragma unroll
-      for (int b = 0; b < FN; ++b)
-        fb[b] = *reinterpret_cast<const float4*>(Bs + (wn + b * 16 + li) * K1P + ks + 4 * lg);
-#pragma unroll
-      for (int a = 0; a < FM; ++a)
-#pragma unroll
-        for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].x, fb[b].x, acc[a][b], 0, 0, 0);
-#pragma unroll
-      for (int a = 0; a < FM; ++a)
-#pragma unroll
-        for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].y, fb[b].y, acc[a][b], 0, 0, 0);
-#pragma unroll
-      for (int a = 0; a < FM; ++a)
-#pragma unroll
-        for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].z, fb[b].z, acc[a][b], 0, 0, 0);
-#pragma unroll
-      for (int a = 0; a < FM; ++a)
-#pragma unroll
-        for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].w, fb[b].w, acc[a][b], 0, 0, 0);
-    }
-  }
-
-  // The panel (MODE 1) overwrites its own A rows: every wave staged them
-  // into LDS before the barrier above, so these stores cannot race.
-  if constexpr (MODE == 0) {
-#pragma unroll
-    for (int a = 0; a < FM; ++a)
-#pragma unroll
-      for (int b = 0; b < FN; ++b)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          if (ok[a][b][v]) gst(Cbase + off[a][b][v], old[a][b][v] - acc[a][b][v]);
-  } else {
-#pragma unroll
-    for (int a = 0; a < FM; ++a)
-#pragma unroll
-      for (int b = 0; b < FN; ++b)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          if (ok[a][b][v]) gst(Cbase + off[a][b][v], acc[a][b][v]);
-  }
-}
-
 // SYRK tile T x T (128, or 64 when a launch has too few 128-tiles to fill
 // the GPU); panel row tile PT (same rule).
 template <int T, bool VEC, bool SPLIT>
@@ -500,24 +277,12 @@ __global__ __launch_bounds__(kThreads) void k_panel_mfma(Batch p, int k0, const 
   gemm_tile<PT, kNB, 1, VEC>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, Lb, M.ld, i0, k0, M.n, k0 + kNB);
 }
 
-// One-shot forms (K = 128) of the two kernels above.
+// One-shot forms (K = 128) of the two kernels above (tiles in chol_once.h).
+
 template <int T, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void k_syrk_once(Batch p, int col0, int col1, int k0) {
-  const int64_t b = blockIdx.y;
-  const Mat M = mat_at(p, b);
-  const int c1 = col1 < 0 ? M.n : col1;
-  if (c1 <= col0) return;
-  const int TI = (M.n - col0 + T - 1) / T;
-  const int TJ = (c1 - col0 + T - 1) / T;
-  const int64_t T64 = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
-  if ((int64_t)blockIdx.x >= T64) return;
-  int ti, tj;
-  decode_tile(xcd_tile(blockIdx.x, (int)T64), TI, ti, tj);
-  float* Lb = M.L;
-  const float* Cs = p.first ? M.A : M.L;
-  const int i0 = col0 + T * ti;
-  const int j0 = col0 + T * tj;
-  gemm_tile_once<T, T, 0, VEC, SPLIT>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, Lb, Cs, M.ld, i0, j0, M.n, c1);
+  __shared__ __align__(16) float lds[once_lds_floats<T, T, SPLIT>()];
+  syrk_once_tile<T, VEC, SPLIT>(p, blockIdx.x, blockIdx.y, col0, col1, k0, p.first, lds);
 }
 
 template <int PT, bool VEC>
@@ -529,16 +294,11 @@ __global__ __launch_bounds__(kThreads) void k_panel_once(Batch p, int k0, const 
   float* Lb = M.L;
   const float* As = p.first ? M.A : M.L;
   const float* Wb = winv + b * (kNB * kNB);
+  __shared__ __align__(16) float lds[once_lds_floats<PT, kNB, false>()];
   gemm_tile_once<PT, kNB, 1, VEC, false>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, Lb, Lb, M.ld, i0, k0, M.n,
-                                         k0 + kNB);
+                                         k0 + kNB, lds);
 }
 
-bool vec_ok(const Batch& p) {
-  if (p.Ap) return p.ld % 4 == 0;  // ragged: the host sets ld = 4 when every matrix is aligned
-  // a first-touch launch also reads A with float4 loads
-  const uintptr_t base = reinterpret_cast<uintptr_t>(p.L) | (p.first ? reinterpret_cast<uintptr_t>(p.A) : 0);
-  return (base & 15) == 0 && (p.ld % 4) == 0 && (p.bs % 4) == 0;
-}
 
 // Workgroups that keep every CU busy with at least two (256 CUs x 2).
 constexpr int64_t kFillWGs = 512;
@@ -557,6 +317,13 @@ void syrk_launch(const Batch& p, int col0, int col1, int k0, int kl, bool vec, h
 }
 
 }  // namespace
+
+bool vec_ok(const Batch& p) {
+  if (p.Ap) return p.ld % 4 == 0;  // ragged: the host sets ld = 4 when every matrix is aligned
+  // a first-touch launch also reads A with float4 loads
+  const uintptr_t base = reinterpret_cast<uintptr_t>(p.L) | (p.first ? reinterpret_cast<uintptr_t>(p.A) : 0);
+  return (base & 15) == 0 && (p.ld % 4) == 0 && (p.bs % 4) == 0;
+}
 
 int launch_panel(const Batch& p, int k0, const float* winv, hipStream_t s) {
   const int rows = p.n - (k0 + kNB);
