@@ -155,11 +155,13 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
         // Right-looking update by this panel: the next column block now (the
         // next diagonal step needs it); the columns beyond it with the next
         // diagonal-block launch, on the CUs its blocks leave idle, when they
-        // fit there in about three rounds.
+        // fit there in a few rounds (fewer while the previous outer block's
+        // bulk update runs on the side stream and takes CUs too).
         const int cn = k0 + 2 * kNB;
+        const int rounds = bulk_pending ? 3 : 8;
         bulk = DiagBulk{cn, E, k0, pk.first, split};
         if (cn < E && !p.Ap && p.batch <= 192 && vec_ok(pk) &&
-            diag_bulk_tiles(p, bulk) <= 3 * (256 - p.batch)) {
+            diag_bulk_tiles(p, bulk) <= (int64_t)rounds * (256 - p.batch)) {
           if ((rc = launch_syrk(pk, k0 + kNB, cn, k0, kNB, split, s))) return rc;
           carry = &bulk;
         } else if ((rc = launch_syrk(pk, k0 + kNB, E, k0, kNB, split, s))) {
