@@ -119,11 +119,45 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
   const int n = p.n;
   // No copy of tril(A) into L: the kernels that first touch a region read
   // it from A (pf).  L's upper triangle is zeroed by extra workgroups of the
-  // first diagonal-block launch where the batch leaves CUs idle, else by
-  // k_zero_upper; each diagonal-block kernel stores its block whole.
+  // diagonal-block launches, one per CU that the last round of the batch
+  // leaves idle (every workgroup of that kernel takes a CU's LDS), each
+  // launch an equal share of the area in consecutive rows, while a
+  // workgroup's share stays under ~160 KiB (one workgroup per CU streams
+  // far slower than the separate fill kernel); else by k_zero_upper.  Each
+  // diagonal-block kernel stores its block whole.
   Batch pf = p;
   pf.first = true;
-  if (n > kNB && zero_upper_wgs(p) == 0 && (rc = launch_zero_upper(p, s))) return rc;
+  const int nblk = (n + kNB - 1) / kNB;
+  const int idle = (int)((256 - p.batch % 256) % 256);
+  const double share = 2.0 * (double)n * (double)n * (double)p.batch / nblk;  // bytes per launch
+  const bool zero_in_diag = n > kNB && idle >= 32 && share <= 160.0 * 1024.0 * idle;
+  std::vector<int> zrow;  // launch j zeros rows [zrow[j], zrow[j + 1])
+  if (zero_in_diag) {
+    // area above the diagonal of rows < R: sum (n - 1 - r) = R (n - 1) - R (R - 1) / 2
+    auto area = [n](int64_t R) { return R * (n - 1) - R * (R - 1) / 2; };
+    const int64_t total = area(n);
+    zrow.resize(nblk + 1);
+    zrow[0] = 0;
+    for (int j = 1; j < nblk; ++j) {
+      const int64_t target = total * j / nblk;
+      int lo = zrow[j - 1], hi = n;  // first R with area(R) >= target
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (area(mid) >= target) hi = mid; else lo = mid + 1;
+      }
+      zrow[j] = lo;
+    }
+    zrow[nblk] = n;
+  } else if (n > kNB && (rc = launch_zero_upper(p, s))) {
+    return rc;
+  }
+  auto zero_of = [&](int k0, DiagZero& z) -> const DiagZero* {
+    if (!zero_in_diag) return nullptr;
+    const int j = k0 / kNB;
+    z = DiagZero{zrow[j], zrow[j + 1], idle};
+    return &z;
+  };
+  DiagZero zcur{};
   bool bulk_pending = false;
   for (int K0 = 0; K0 < n; K0 += nbo) {
     const int W = (n - K0) < nbo ? (n - K0) : nbo;
@@ -147,7 +181,7 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
     for (int k0 = K0; k0 < E; k0 += kNB) {
       const Batch& pk = k0 == 0 ? pf : p;
       if (left && k0 > K0 && (rc = launch_syrk(p0, k0, k0 + kNB, K0, k0 - K0, split, s))) return rc;
-      if ((rc = launch_diag_factor(pk, k0, winv, s, carry))) return rc;
+      if ((rc = launch_diag_factor(pk, k0, winv, s, carry, zero_of(k0, zcur)))) return rc;
       carry = nullptr;
       if (k0 + kNB >= n) break;
       if ((rc = launch_panel(pk, k0, winv, s))) return rc;

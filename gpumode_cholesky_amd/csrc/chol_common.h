@@ -255,7 +255,6 @@ int launch_lds(const Batch& p, hipStream_t s);                    // n <= 256
 int launch_copy_lower(const Batch& p, hipStream_t s);             // L = tril(A)
 int launch_zero_upper(const Batch& p, hipStream_t s);             // L = 0 right of the diagonal chunks
 bool vec_ok(const Batch& p);  // float4 global access allowed for every matrix of p
-int zero_upper_wgs(const Batch& p);  // zero-fill workgroups added to the first diagonal-block launch (0: none)
 // One-shot (K = kNB) SYRK tiles that a diagonal-block launch runs on the
 // CUs its blocks leave idle: C -= P P^T for columns [c0, c1), P = the kNB
 // columns from k0; first: old values of C from A.
@@ -264,8 +263,14 @@ struct DiagBulk {
   bool first, split;
 };
 int64_t diag_bulk_tiles(const Batch& p, const DiagBulk& u);  // workgroups it adds (tiles x batch)
-int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s,
-                       const DiagBulk* bulk = nullptr);           // kNB x kNB block
+// A share of the zero fill of L's upper triangle carried by a
+// diagonal-block launch: rows [r0, r1) of every matrix, by `wgs` extra
+// workgroups.
+struct DiagZero {
+  int r0, r1, wgs;
+};
+int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const DiagBulk* bulk = nullptr,
+                       const DiagZero* zero = nullptr);           // kNB x kNB block
 int launch_panel(const Batch& p, int k0, const float* winv,
                  hipStream_t s);                                  // L21 = A21 W^T
 // C -= P P^T over {col0 <= j < col1, j <= i < n}, K = [k0, k0+kl);

@@ -331,21 +331,22 @@ __device__ __forceinline__ void w21_tiles(const float* Ls, float* Ws, floatx4 (&
   U[1] = mma_nr(L3, P, acc[1], U[1], li, lg);
 }
 
-// L = 0 above the diagonal, in whole 4-column chunks right of each row's
-// diagonal chunk (the diagonal-block kernels store the chunks that hold a
-// diagonal element), by `nwg` workgroups, streaming stores.
-__device__ __forceinline__ void zero_upper(const Batch& p, int wg, int nwg) {
+// L = 0 above the diagonal in rows [r0, r1) of every matrix, in whole
+// 4-column chunks right of each row's diagonal chunk (the diagonal-block
+// kernels store the chunks that hold a diagonal element), by `nwg`
+// workgroups, streaming stores.
+__device__ __forceinline__ void zero_upper_rows(const Batch& p, int r0, int r1, int wg, int nwg) {
   const int n = p.n;
   const int n4 = (n + 3) / 4;
   const bool vec = ((reinterpret_cast<uintptr_t>(p.L) & 15) == 0) && (p.ld % 4) == 0 && (p.bs % 4) == 0 &&
                    (n % 4) == 0;
-  const int64_t per = (int64_t)n * n4;
+  const int64_t per = (int64_t)(r1 - r0) * n4;
   const int64_t total = per * p.batch;
   for (int64_t e = (int64_t)wg * 256 + threadIdx.x; e < total; e += (int64_t)nwg * 256) {
     const int64_t b = e / per;
     const int64_t r = e - b * per;
-    const int i = (int)(r / n4);
-    const int c = 4 * (int)(r - (int64_t)i * n4);
+    const int i = r0 + (int)(r / n4);
+    const int c = 4 * (int)(r - (int64_t)(i - r0) * n4);
     if (c <= i) continue;
     float* q = p.L + b * p.bs + (int64_t)i * p.ld + c;
     if (vec) {
@@ -356,21 +357,25 @@ __device__ __forceinline__ void zero_upper(const Batch& p, int wg, int nwg) {
   }
 }
 
-// BULK: 0 none, 1 FP32, 2 BF16x3 SYRK tiles (64x64, one-shot) in the
-// workgroups past the batch (bt tiles per matrix).
+// Workgroups past the batch: nz of them zero L's upper triangle in rows
+// [zr0, zr1); then, BULK (1 FP32, 2 BF16x3), the previous step's
+// trailing-update tiles (64x64, one-shot; bt per matrix).
 template <bool FROM_A, bool LAST, int BULK>
 __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv, int bc0, int bc1,
-                                                     int bk0, int bt, int bfirst) {
+                                                     int bk0, int bt, int bfirst, int zr0, int zr1, int nz) {
   __shared__ DiagLds sm;
   // Workgroups past the batch run on the CUs that the diagonal blocks leave
-  // idle: the first block of a dense matrix zeros L's upper triangle; a
-  // later one runs the previous step's trailing-update tiles beyond the
-  // next column block (which no diagonal block of this launch reads).
+  // idle: a share of the zero fill of L's upper triangle, and the previous
+  // step's trailing-update tiles beyond the next column block (which no
+  // diagonal block of this launch reads).
   if (blockIdx.x >= p.batch) {
-    const int id = blockIdx.x - (int)p.batch;
-    if constexpr (FROM_A) {
-      zero_upper(p, id, gridDim.x - (int)p.batch);
-    } else if constexpr (BULK != 0) {
+    int id = blockIdx.x - (int)p.batch;
+    if (id < nz) {
+      zero_upper_rows(p, zr0, zr1, id, nz);
+      return;
+    }
+    id -= nz;
+    if constexpr (BULK != 0) {
       syrk_once_tile<64, true, BULK == 2>(p, id % bt, id / bt, bc0, bc1, bk0, bfirst != 0,
                                           reinterpret_cast<float*>(&sm));
     }
@@ -668,24 +673,14 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 
 }  // namespace
 
-// Zero-fill workgroups for the first diagonal-block launch: one per CU the
-// batch leaves idle, and only while each has at most ~160 KiB to zero (one
-// workgroup per CU streams far less than the separate fill kernel, and the
-// fill must end within the diagonal block's time).
-int zero_upper_wgs(const Batch& p) {
-  if (p.batch > 192) return 0;
-  const int wgs = 256 - (int)p.batch;
-  const double bytes = 2.0 * (double)p.n * (double)p.n * (double)p.batch;
-  return bytes <= 160.0 * 1024.0 * wgs ? wgs : 0;
-}
-
 int64_t diag_bulk_tiles(const Batch& p, const DiagBulk& u) {
   const int TI = (p.n - u.c0 + 63) / 64;
   const int TJ = (u.c1 - u.c0 + 63) / 64;
   return ((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) * p.batch;
 }
 
-int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const DiagBulk* bulk) {
+int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const DiagBulk* bulk,
+                       const DiagZero* zero) {
   // The block that reaches column n needs no inverse (no panel follows).
   // p.first (dense, k0 = 0): staged from A.  (Ragged batches: every matrix
   // must end at this block for LAST; they never take p.first.)
@@ -693,10 +688,13 @@ int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const
   const bool from_a = p.first && !p.Ap;
   const dim3 b(256);
   dim3 g((unsigned)p.batch);
-  // The first block of a dense matrix larger than one block also zeros L's
-  // upper triangle, with one extra workgroup per CU that the batch leaves
-  // idle (every workgroup of this kernel takes a CU's LDS).
-  if (from_a && !last) g.x += (unsigned)zero_upper_wgs(p);
+  int zr0 = 0, zr1 = 0, nz = 0;
+  if (zero && zero->r1 > zero->r0 && zero->wgs > 0) {
+    zr0 = zero->r0;
+    zr1 = zero->r1;
+    nz = zero->wgs;
+    g.x += (unsigned)nz;
+  }
   int bc0 = 0, bc1 = 0, bk0 = 0, bt = 1, bfirst = 0;
   if (bulk) {  // dense, float4-aligned (vec_ok), not the first block
     const int TI = (p.n - bulk->c0 + 63) / 64;
@@ -709,7 +707,7 @@ int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const
     g.x += (unsigned)(bt * p.batch);
   }
 #define CHOL_DIAG_LAUNCH(FA, LA, BU) \
-  hipLaunchKernelGGL((k_diag_factor<FA, LA, BU>), g, b, 0, s, p, k0, winv, bc0, bc1, bk0, bt, bfirst)
+  hipLaunchKernelGGL((k_diag_factor<FA, LA, BU>), g, b, 0, s, p, k0, winv, bc0, bc1, bk0, bt, bfirst, zr0, zr1, nz)
   const int bu = bulk ? (bulk->split ? 2 : 1) : 0;
   if (from_a && last) {
     CHOL_DIAG_LAUNCH(true, true, 0);
