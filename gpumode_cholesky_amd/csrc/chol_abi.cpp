@@ -184,7 +184,7 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
       if ((rc = launch_diag_factor(pk, k0, winv, s, carry, zero_of(k0, zcur)))) return rc;
       carry = nullptr;
       if (k0 + kNB >= n) break;
-      if ((rc = launch_panel(pk, k0, winv, s))) return rc;
+      if ((rc = launch_panel(pk, k0, winv, split, s))) return rc;
       if (!left && k0 + kNB < E) {
         // Right-looking update by this panel: the next column block now (the
         // next diagonal step needs it); the columns beyond it with the next

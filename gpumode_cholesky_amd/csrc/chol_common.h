@@ -271,8 +271,8 @@ struct DiagZero {
 };
 int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const DiagBulk* bulk = nullptr,
                        const DiagZero* zero = nullptr);           // kNB x kNB block
-int launch_panel(const Batch& p, int k0, const float* winv,
-                 hipStream_t s);                                  // L21 = A21 W^T
+int launch_panel(const Batch& p, int k0, const float* winv, bool split,
+                 hipStream_t s);  // L21 = A21 W^T (split: BF16x3 products)
 // C -= P P^T over {col0 <= j < col1, j <= i < n}, K = [k0, k0+kl);
 // split = BF16x3 products (mixed mode) instead of FP32.
 int launch_syrk(const Batch& p, int col0, int col1, int k0, int kl, bool split,

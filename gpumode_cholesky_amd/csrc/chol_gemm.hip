@@ -166,8 +166,9 @@ __device__ __forceinline__ void gemm_tile(const float* Abase, int64_t lda, int a
 // The dropped loA loB^T term and the split remainder are ~2^-16 of |a b|.
 // Operand layout of 16x16x32 bf16: A[i][k] for k = 8g..8g+7 in lane i + 16g,
 // B likewise by column; D as for 16x16x4 f32.
-// C -= A B^T on the lower part (i >= j), j < jlim, i < ilim, BF16x3 products.
-template <int BM, int BN, bool VEC>
+// MODE 0: C -= A B^T on the lower part (i >= j), j < jlim, i < ilim;
+// MODE 1: C = A B^T, i < ilim (panel); BF16x3 products.
+template <int BM, int BN, bool VEC, int MODE = 0>
 __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda, int a_row0, int a_lim, int ka,
                                                  const float* Bbase, int64_t ldb, int b_row0, int b_lim, int kb,
                                                  int K, float* Cbase, const float* Csrc, int64_t ldc, int c_row0,
@@ -238,7 +239,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda
     __syncthreads();
   }
 
-  epilogue<FM, FN, 0>(acc, Cbase, Csrc, ldc, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
+  epilogue<FM, FN, MODE>(acc, Cbase, Csrc, ldc, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
 }
 
 // SYRK tile T x T (128, or 64 when a launch has too few 128-tiles to fill
@@ -265,7 +266,8 @@ __global__ __launch_bounds__(kThreads) void k_syrk_mfma(Batch p, int col0, int c
     gemm_tile<T, T, 0, VEC>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, kl, Lb, Cs, M.ld, i0, j0, M.n, c1);
 }
 
-template <int PT, bool VEC>
+// Panel: L21 = A21 W^T (SPLIT: BF16x3 products, the mixed form).
+template <int PT, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void k_panel_mfma(Batch p, int k0, const float* __restrict__ winv) {
   const int64_t b = blockIdx.y;
   const Mat M = mat_at(p, b);
@@ -274,7 +276,11 @@ __global__ __launch_bounds__(kThreads) void k_panel_mfma(Batch p, int k0, const 
   float* Lb = M.L;
   const float* As = p.first ? M.A : M.L;
   const float* Wb = winv + b * (kNB * kNB);
-  gemm_tile<PT, kNB, 1, VEC>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, Lb, M.ld, i0, k0, M.n, k0 + kNB);
+  if constexpr (SPLIT)
+    gemm_tile_bf16x3<PT, kNB, VEC, 1>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, Lb, M.ld, i0, k0, M.n,
+                                      k0 + kNB);
+  else
+    gemm_tile<PT, kNB, 1, VEC>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, Lb, M.ld, i0, k0, M.n, k0 + kNB);
 }
 
 // One-shot forms (K = 128) of the two kernels above (tiles in chol_once.h).
@@ -285,7 +291,7 @@ __global__ __launch_bounds__(kThreads) void k_syrk_once(Batch p, int col0, int c
   syrk_once_tile<T, VEC, SPLIT>(p, blockIdx.x, blockIdx.y, col0, col1, k0, p.first, lds);
 }
 
-template <int PT, bool VEC>
+template <int PT, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void k_panel_once(Batch p, int k0, const float* __restrict__ winv) {
   const int64_t b = blockIdx.y;
   const Mat M = mat_at(p, b);
@@ -294,8 +300,8 @@ __global__ __launch_bounds__(kThreads) void k_panel_once(Batch p, int k0, const 
   float* Lb = M.L;
   const float* As = p.first ? M.A : M.L;
   const float* Wb = winv + b * (kNB * kNB);
-  __shared__ __align__(16) float lds[once_lds_floats<PT, kNB, false>()];
-  gemm_tile_once<PT, kNB, 1, VEC, false>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, Lb, Lb, M.ld, i0, k0, M.n,
+  __shared__ __align__(16) float lds[once_lds_floats<PT, kNB, SPLIT>()];
+  gemm_tile_once<PT, kNB, 1, VEC, SPLIT>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, Lb, Lb, M.ld, i0, k0, M.n,
                                          k0 + kNB, lds);
 }
 
@@ -325,7 +331,7 @@ bool vec_ok(const Batch& p) {
   return (base & 15) == 0 && (p.ld % 4) == 0 && (p.bs % 4) == 0;
 }
 
-int launch_panel(const Batch& p, int k0, const float* winv, hipStream_t s) {
+int launch_panel(const Batch& p, int k0, const float* winv, bool split, hipStream_t s) {
   const int rows = p.n - (k0 + kNB);
   if (rows <= 0) return 0;
   // One-shot 32-row tiles (W, 64 KiB, and the rows staged in one pass)
@@ -333,19 +339,23 @@ int launch_panel(const Batch& p, int k0, const float* winv, hipStream_t s) {
   // beyond that the pipelined 128-row tiles, which stage W once per 128 rows.
   const bool small = (int64_t)((rows + 127) / 128) * p.batch <= 64;
   const bool vec = vec_ok(p);
-  if (small) {
-    dim3 grid((unsigned)((rows + 31) / 32), (unsigned)p.batch);
-    if (vec)
-      hipLaunchKernelGGL((k_panel_once<32, true>), grid, dim3(kThreads), 0, s, p, k0, winv);
-    else
-      hipLaunchKernelGGL((k_panel_once<32, false>), grid, dim3(kThreads), 0, s, p, k0, winv);
-  } else {
-    dim3 grid((unsigned)((rows + 127) / 128), (unsigned)p.batch);
-    if (vec)
-      hipLaunchKernelGGL((k_panel_mfma<128, true>), grid, dim3(kThreads), 0, s, p, k0, winv);
-    else
-      hipLaunchKernelGGL((k_panel_mfma<128, false>), grid, dim3(kThreads), 0, s, p, k0, winv);
+#define CHOL_PANEL(KERNEL, PT, ROWS_PER)                                                                        \
+  {                                                                                                            \
+    dim3 grid((unsigned)((rows + (ROWS_PER)-1) / (ROWS_PER)), (unsigned)p.batch);                              \
+    if (vec && split)                                                                                          \
+      hipLaunchKernelGGL((KERNEL<PT, true, true>), grid, dim3(kThreads), 0, s, p, k0, winv);                   \
+    else if (vec)                                                                                              \
+      hipLaunchKernelGGL((KERNEL<PT, true, false>), grid, dim3(kThreads), 0, s, p, k0, winv);                  \
+    else if (split)                                                                                            \
+      hipLaunchKernelGGL((KERNEL<PT, false, true>), grid, dim3(kThreads), 0, s, p, k0, winv);                  \
+    else                                                                                                       \
+      hipLaunchKernelGGL((KERNEL<PT, false, false>), grid, dim3(kThreads), 0, s, p, k0, winv);                 \
   }
+  if (small)
+    CHOL_PANEL(k_panel_once, 32, 32)
+  else
+    CHOL_PANEL(k_panel_mfma, 128, 128)
+#undef CHOL_PANEL
   return launch_status("k_panel_mfma");
 }
 

@@ -101,14 +101,13 @@ constexpr int once_lds_floats() {
 
 // MODE 0: C -= A B^T on the lower part (i >= j), i < ilim, j < jlim.
 // MODE 1: C  = A B^T, i < ilim (panel; A and C alias row-for-row).
-// SPLIT: BF16x3 products (x = hi + lo, hi*hi + hi*lo + lo*hi; MODE 0 only).
+// SPLIT: BF16x3 products (x = hi + lo, hi*hi + hi*lo + lo*hi).
 
 template <int BM, int BN, int MODE, bool VEC, bool SPLIT>
 __device__ __forceinline__ void gemm_tile_once(const float* Abase, int64_t lda, int a_row0, int a_lim, int ka,
                                                const float* Bbase, int64_t ldb, int b_row0, int b_lim, int kb,
                                                float* Cbase, const float* Csrc, int64_t ldc, int c_row0,
                                                int c_col0, int c_ilim, int c_jlim, float* __restrict__ lds) {
-  static_assert(!SPLIT || MODE == 0, "split products are used for the trailing update only");
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int PA = BM * K1 / 4 / kThreads;

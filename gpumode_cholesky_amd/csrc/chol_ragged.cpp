@@ -155,7 +155,7 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
       if ((rc = launch_diag_factor(q, k0, winv, s))) return rc;
       if (act2 == 0) break;
       q.batch = act2;
-      if ((rc = launch_panel(q, k0, winv, s))) return rc;
+      if ((rc = launch_panel(q, k0, winv, mode == CHOL_MODE_MIXED, s))) return rc;
       if ((rc = launch_syrk(q, k0 + kNB, -1, k0, kNB, mode == CHOL_MODE_MIXED, s))) return rc;
     }
   }
