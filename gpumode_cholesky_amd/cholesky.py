@@ -52,7 +52,9 @@ def cholesky(A: torch.Tensor, mode: str = "plain", return_info: bool = False,
     """Lower Cholesky factor of each matrix of ``A`` ([..., n, n] FP32).
 
     mode: ``"plain"`` (all-FP32) or ``"mixed"`` (BF16x3 split products in the
-    trailing updates; differs from plain for n > 256 only).
+    panel solves and trailing updates of the blocked path, which runs for
+    n > 256 and for 64 < n <= 256 where the tuning table picks it; the
+    register and LDS kernels are the same in both forms).
     Returns ``L`` or ``(L, info)`` with ``info`` int32 [...] on the device.
     """
     m = _MODES.get(mode)

@@ -42,7 +42,7 @@ extern "C" {
 
 /* Precision modes. */
 #define CHOL_MODE_PLAIN 0 /* all-FP32 arithmetic                            */
-#define CHOL_MODE_MIXED 1 /* BF16x3 split-product trailing updates, FP32 acc */
+#define CHOL_MODE_MIXED 1 /* BF16x3 split-product panels and trailing updates, FP32 acc */
 
 /* Version of this ABI (CHOL_ABI_VERSION). */
 int chol_abi_version(void);
