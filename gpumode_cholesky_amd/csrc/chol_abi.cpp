@@ -127,6 +127,8 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
   // diagonal-block kernel stores its block whole.
   Batch pf = p;
   pf.first = true;
+  bool bulk_pending = false;  // an outer-block bulk update on the side stream
+  bool side_pending = false;  // the zero fill on the side stream
   const int nblk = (n + kNB - 1) / kNB;
   const int idle = (int)((256 - p.batch % 256) % 256);
   const double share = 2.0 * (double)n * (double)n * (double)p.batch / nblk;  // bytes per launch
@@ -148,8 +150,14 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
       zrow[j] = lo;
     }
     zrow[nblk] = n;
-  } else if (n > kNB && (rc = launch_zero_upper(p, s))) {
-    return rc;
+  } else if (n > kNB) {
+    // on the side stream, overlapping the first diagonal blocks: it only
+    // writes zeros where no kernel of the factorisation reads, and the
+    // diagonal-block kernels store the same zeros in their own blocks
+    if ((rc = order(side->to_side, s, side->s2))) return rc;
+    if ((rc = launch_zero_upper(p, side->s2))) return rc;
+    if (hipEventRecord(side->from_side, side->s2) != hipSuccess) return launch_status("hipEventRecord");
+    side_pending = true;
   }
   auto zero_of = [&](int k0, DiagZero& z) -> const DiagZero* {
     if (!zero_in_diag) return nullptr;
@@ -158,7 +166,6 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
     return &z;
   };
   DiagZero zcur{};
-  bool bulk_pending = false;
   for (int K0 = 0; K0 < n; K0 += nbo) {
     const int W = (n - K0) < nbo ? (n - K0) : nbo;
     const int E = K0 + W;  // end of this outer block
@@ -218,7 +225,8 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
       bulk_pending = true;
     }
   }
-  if (bulk_pending && hipStreamWaitEvent(s, side->from_side, 0) != hipSuccess) return launch_status("hipStreamWaitEvent");
+  if ((bulk_pending || side_pending) && hipStreamWaitEvent(s, side->from_side, 0) != hipSuccess)
+    return launch_status("hipStreamWaitEvent");
   return 0;
 }
 
