@@ -1,13 +1,17 @@
 // chol_abi.cpp — the extern "C" boundary (include/chol_amd.h): argument
 // validation, workspace sizing and dispatch to the gfx950 kernels.
 //
-// Dispatch by order n (the reference dispatches by (batch, n) tables in
-// custom_kernel, submission.py:4101-4268; here the route depends only on n so
-// every (batch, n) takes a HIP route and there is no library fallback):
-//   n <= 64        : chol_small.hip, whole matrix in one wavefront's registers;
-//   64 < n <= 256  : chol_lds.hip, whole matrix in one workgroup's LDS;
-//   n >  256       : chol_blocked.hip + chol_gemm.hip, 64-wide diagonal
-//                    blocks with explicit inverses, MFMA panel and SYRK.
+// Dispatch by order n and the tuning table (the reference dispatches by
+// (batch, n) tables in custom_kernel, submission.py:4101-4268; here every
+// (batch, n) takes a HIP route and there is no library fallback):
+//   n <= 64        : chol_small.hip, whole matrices in wavefront registers /
+//                    LDS;
+//   64 < n <= 256  : chol_lds.hip, whole matrix in one workgroup's LDS, or
+//                    the blocked path where the tuning table says so;
+//   n >  256       : the blocked path: 128-wide diagonal blocks with explicit
+//                    inverses (chol_diag.hip), MFMA panels and SYRK
+//                    (chol_gemm.hip, chol_syrk256.hip), driven by
+//                    blocked_potrf below.
 #include <cstdarg>
 #include <cstring>
 #include <mutex>

@@ -1,10 +1,11 @@
-// chol_gemm.hip — FP32 MFMA products of the blocked path (gfx950).
+// chol_gemm.hip — MFMA products of the blocked path (gfx950).
 //
 //   syrk  : C[i][j] -= sum_k P[i][k] P[j][k]   over the lower trailing tiles
-//   panel : L21 = A21 W^T                        (in place, K = 64)
+//   panel : L21 = A21 W^T                        (in place, K = 128)
 //
 // Both are C (op)= A B^T with A, B row-major along K, run on
-// v_mfma_f32_16x16x4_f32.  A workgroup of 4 wavefronts computes a BM x BN
+// v_mfma_f32_16x16x4_f32 (plain) or as BF16x3 split products on
+// v_mfma_f32_16x16x32_bf16 (mixed).  A workgroup of 4 wavefronts computes a BM x BN
 // tile; each wavefront a (BM/2) x (BN/2) quarter as 16x16 MFMA blocks.  K is
 // staged through LDS in chunks of 32 (rows padded to 36 floats), double
 // buffered: the global loads of chunk c+1 are in flight while chunk c is
@@ -18,7 +19,7 @@
 // Reference counterparts: the trailing baddbmm_ / addmm_ SYRK-as-GEMM and
 // the panel-times-inverse bmm of the reference's blocked drivers
 // (submission.py:5163-5184, :2529-2534, :3362-3401), which ran on TF32/FP16
-// tensor cores; here all-FP32 matrix cores.
+// tensor cores; here FP32 or BF16x3 on the matrix cores.
 #include "chol_common.h"
 #include "chol_once.h"
 
