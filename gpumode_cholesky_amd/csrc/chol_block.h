@@ -23,19 +23,9 @@ __device__ __forceinline__ floatx4_b mma_nt_p(const float* X, const float* Y, fl
   return acc;
 }
 
-// One wavefront factors the 64x64 block of Ls (pitch P) at (d, d), blocked by PW
-// columns: for each column panel [c0, c0+PW), lane = row,
-//   1. reg_factor_rl<PW> on rows c0..d+63 yields the panel's diagonal factor
-//      and, for the rows below it, its part of L (the panel's own
-//      triangular solve), broadcasting pivot columns with v_readlane;
-//   2. the lower 16x16 tiles of the trailing block take the panel's rank-PW
-//      update on v_mfma_f32_16x16x4_f32.
-// A pivot step then costs a few dependent VALU operations plus
-// 2 (PW-1-k) issue slots, where a 64-wide register chain spends an LDS
-// round trip and 63-k row updates.  Returns the 1-based first failing
-// pivot (0 if none).
-// One column panel s of factor64: the register chain on rows c0..d+63 and
-// the panel's trailing update.  Accumulates the 1-based failing pivot in bad.
+// One column panel S of factor64 (below): the register chain on rows
+// c0..d+63 (c0 = d + PW S) and the panel's trailing update.  Accumulates the
+// 1-based failing pivot in bad.
 template <int PW, int P, int S>
 __device__ __forceinline__ void factor64_step(float* Ls, int d, int lane, int& bad) {
   const int li = lane & 15, lg = lane >> 4;
