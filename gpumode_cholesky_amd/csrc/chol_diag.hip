@@ -562,7 +562,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int e = lane + 64 * u;
-      gst4(&Wb[(r0 + (e >> 5)) * NB + (e & 31) * 4], v[u]);
+      gst4_nt(&Wb[(r0 + (e >> 5)) * NB + (e & 31) * 4], v[u]);
     }
   };
 
