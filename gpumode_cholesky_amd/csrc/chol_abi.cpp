@@ -204,9 +204,18 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
         // bulk update runs on the side stream and takes CUs too).
         const int cn = k0 + 2 * kNB;
         const int rounds = bulk_pending ? 3 : 8;
+        const bool hand_in = !p.Ap && p.batch <= 192 && vec_ok(pk);
+        if (split && hand_in) {
+          // mixed form: the whole update, the next block's own part included,
+          // rides on the next diagonal-block launch
+          bulk = DiagBulk{k0 + kNB, E, k0, pk.first, split, true};
+          if (diag_bulk_tiles(p, bulk) <= (int64_t)rounds * (256 - p.batch)) {
+            carry = &bulk;
+            continue;
+          }
+        }
         bulk = DiagBulk{cn, E, k0, pk.first, split};
-        if (cn < E && !p.Ap && p.batch <= 192 && vec_ok(pk) &&
-            diag_bulk_tiles(p, bulk) <= (int64_t)rounds * (256 - p.batch)) {
+        if (cn < E && hand_in && diag_bulk_tiles(p, bulk) <= (int64_t)rounds * (256 - p.batch)) {
           if ((rc = launch_syrk(pk, k0 + kNB, cn, k0, kNB, split, s))) return rc;
           carry = &bulk;
         } else if ((rc = launch_syrk(pk, k0 + kNB, E, k0, kNB, split, s))) {

@@ -257,10 +257,13 @@ int launch_zero_upper(const Batch& p, hipStream_t s);             // L = 0 right
 bool vec_ok(const Batch& p);  // float4 global access allowed for every matrix of p
 // One-shot (K = kNB) SYRK tiles that a diagonal-block launch runs on the
 // CUs its blocks leave idle: C -= P P^T for columns [c0, c1), P = the kNB
-// columns from k0; first: old values of C from A.
+// columns from k0; first: old values of C from A; pre (split only, c0 =
+// the launch's own block): the launch's block is one of those columns and
+// its kernel applies its share of the update itself before factoring it.
 struct DiagBulk {
   int c0, c1, k0;
   bool first, split;
+  bool pre = false;
 };
 int64_t diag_bulk_tiles(const Batch& p, const DiagBulk& u);  // workgroups it adds (tiles x batch)
 // A share of the zero fill of L's upper triangle carried by a

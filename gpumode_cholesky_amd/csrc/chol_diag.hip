@@ -50,6 +50,8 @@ __device__ __forceinline__ floatx4 mma_nn(const float* X, int px, const float* Y
 }
 
 constexpr int XP = 16 + 4;  // pitch of the L21 panel slabs
+constexpr int PH = NB + 8;  // BF16 pitch of a staged previous-panel block (PRE)
+typedef __bf16 bf16x4d __attribute__((ext_vector_type(4)));
 
 // LDS of the kernel: the staging tiles first, then the block and its
 // inverse (Ls starts below 64 KiB, so most of its accesses encode their
@@ -60,6 +62,9 @@ struct __align__(16) DiagLds {
   float Ws[NB * P];
   float Xp[2][H * XP];  // waves 1, 2: their copy of the current L21 panel
 };
+// PRE: a previous-panel block as BF16 hi / lo fills Ws and runs into Xp,
+// which is unused until then.
+static_assert(2 * NB * PH * 2 <= (NB * P + 2 * H * XP) * 4, "staged BF16 block exceeds Ws + Xp");
 
 // C(16x16) += X Y^T, rows of X (pitch px) and Y (pitch py) read as float4, K = 16.
 __device__ __forceinline__ floatx4 mma_nt2(const float* X, int px, const float* Y, int py, floatx4 acc, int li,
@@ -357,10 +362,61 @@ __device__ __forceinline__ void zero_upper_rows(const Batch& p, int r0, int r1, 
   }
 }
 
+// PRE (mixed form): D -= P P^T on lower 16x16 tiles of the block in Ls, P
+// (128 x 128, the previous panel's rows of this block) staged as BF16 hi /
+// lo (pitch PH) over Ws and Xp; BF16x3 products as in the SYRK tiles.  The
+// tiles (TA, j), JA0 <= j < JA1, and (TB, j), JB0 <= j < JB1: independent
+// accumulators, so the matrix-core chains of the tiles interleave.
+template <int TA, int JA0, int JA1, int TB, int JB0, int JB1>
+__device__ __forceinline__ void pre_tiles(float* Ls, const __bf16* Ph, int li, int lg) {
+  constexpr int NA = JA1 - JA0;
+  constexpr int NT = NA + (JB1 - JB0);
+  const __bf16* Pl = Ph + NB * PH;
+  floatx4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const int ko = 32 * kk + 8 * lg;
+    const bf16x8 aah = *reinterpret_cast<const bf16x8*>(Ph + (16 * TA + li) * PH + ko);
+    const bf16x8 aal = *reinterpret_cast<const bf16x8*>(Pl + (16 * TA + li) * PH + ko);
+    const bf16x8 abh = *reinterpret_cast<const bf16x8*>(Ph + (16 * TB + li) * PH + ko);
+    const bf16x8 abl = *reinterpret_cast<const bf16x8*>(Pl + (16 * TB + li) * PH + ko);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int tj = t < NA ? JA0 + t : JB0 + t - NA;
+      const bf16x8 ah = t < NA ? aah : abh;
+      const bf16x8 al = t < NA ? aal : abl;
+      const bf16x8 bh = *reinterpret_cast<const bf16x8*>(Ph + (16 * tj + li) * PH + ko);
+      const bf16x8 bl = *reinterpret_cast<const bf16x8*>(Pl + (16 * tj + li) * PH + ko);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[t], 0, 0, 0);
+    }
+  }
+  float old[NT][4];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int ti = t < NA ? TA : TB, tj = t < NA ? JA0 + t : JB0 + t - NA;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) old[t][v] = Ls[(16 * ti + 4 * lg + v) * P + 16 * tj + li];
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int ti = t < NA ? TA : TB, tj = t < NA ? JA0 + t : JB0 + t - NA;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (ti != tj || li <= 4 * lg + v) Ls[(16 * ti + 4 * lg + v) * P + 16 * tj + li] = old[t][v] - acc[t][v];
+  }
+}
+
 // Workgroups past the batch: nz of them zero L's upper triangle in rows
 // [zr0, zr1); then, BULK (1 FP32, 2 BF16x3), the previous step's
 // trailing-update tiles (64x64, one-shot; bt per matrix).
-template <bool FROM_A, bool LAST, int BULK>
+// PRE (with BULK == 2): the previous panel (columns bk0..bk0+127) has not
+// yet updated this block: the kernel applies that update to it first, and
+// the workgroups past the batch (skipping this block) the rest of it.
+template <bool FROM_A, bool LAST, int BULK, bool PRE = false>
 __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv, int bc0, int bc1,
                                                      int bk0, int bt, int bfirst, int zr0, int zr1, int nz) {
   __shared__ DiagLds sm;
@@ -377,7 +433,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     id -= nz;
     if constexpr (BULK != 0) {
       syrk_once_tile<64, true, BULK == 2>(p, id % bt, id / bt, bc0, bc1, bk0, bfirst != 0,
-                                          reinterpret_cast<float*>(&sm));
+                                          reinterpret_cast<float*>(&sm), PRE);
     }
     return;
   }
@@ -395,10 +451,10 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   if (nb <= 0) return;  // (ragged) this matrix has no block at k0
   float* Lb = M.L + (int64_t)k0 * M.ld + k0;
   // FROM_A: the first block, staged from A (nothing has written L yet).
-  const float* Sb = FROM_A ? M.A + (int64_t)k0 * M.ld + k0 : Lb;
+  const bool from_a = FROM_A || (PRE && bfirst != 0);
+  const float* Sb = from_a ? M.A + (int64_t)k0 * M.ld + k0 : Lb;
   const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0) &&
-                   (!FROM_A || (reinterpret_cast<uintptr_t>(M.A) & 15) == 0);
-
+                   (!from_a || (reinterpret_cast<uintptr_t>(M.A) & 15) == 0);
   // ---- 1. stage: thread covers rows tid/32 + 8u, columns 4*(tid%32).
   //         All 16 loads are issued before the first LDS store, so the
   //         block costs one memory latency, not one per few rows. ----
@@ -451,13 +507,52 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     stg[u] = v;
   }
   }
+  // PRE: the previous panel's rows of this block, requested right behind
+  // the block (rows 0..63 are needed first: u < 8)
+  float4 pst[PRE ? NB / 8 : 1];
+  __bf16* const Ph = reinterpret_cast<__bf16*>(Ws);
+  if constexpr (PRE) {
+    const float* Pb = M.L + (int64_t)k0 * M.ld + bk0;
+#pragma unroll
+    for (int u = 0; u < NB / 8; ++u) {
+      const int r = (tid >> 5) + 8 * u;
+      pst[u] = r < nb ? gld4(Pb + (int64_t)r * M.ld + c4) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  auto put_p = [&](int u) {  // P row tid/32 + 8u as BF16 hi / lo
+    const int r = (tid >> 5) + 8 * u;
+    const float4 v = pst[u];
+    const __bf16 h0 = (__bf16)v.x, h1 = (__bf16)v.y, h2 = (__bf16)v.z, h3 = (__bf16)v.w;
+    const bf16x4d hi = {h0, h1, h2, h3};
+    const bf16x4d lo = {(__bf16)(v.x - (float)h0), (__bf16)(v.y - (float)h1), (__bf16)(v.z - (float)h2),
+                        (__bf16)(v.w - (float)h3)};
+    *reinterpret_cast<bf16x4d*>(Ph + r * PH + c4) = hi;
+    *reinterpret_cast<bf16x4d*>(Ph + NB * PH + r * PH + c4) = lo;
+  };
 #pragma unroll
   for (int u = 0; u < NB / 8; ++u) {
     const int r = (tid >> 5) + 8 * u;
     *reinterpret_cast<float4*>(&Ls[r * P + c4]) = stg[u];
-    if constexpr (!LAST) *reinterpret_cast<float4*>(&Ws[r * P + c4]) = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (PRE) {
+      if (u < NB / 16) put_p(u);
+    }
   }
   __syncthreads();
+  if constexpr (PRE) {
+    // D1's ten tiles (P rows 0..63) by all four wavefronts, then the rest
+    // of P into LDS; D21 and D2 (rows 64..127) follow in D1's first step.
+    if (wave == 0)
+      pre_tiles<0, 0, 1, 1, 0, 2>(Ls, Ph, li, lg);
+    else if (wave == 1)
+      pre_tiles<2, 0, 3, 0, 0, 0>(Ls, Ph, li, lg);
+    else if (wave == 2)
+      pre_tiles<3, 0, 2, 0, 0, 0>(Ls, Ph, li, lg);
+    else
+      pre_tiles<3, 2, 4, 0, 0, 0>(Ls, Ph, li, lg);
+#pragma unroll
+    for (int u = NB / 16; u < NB / 8; ++u) put_p(u);
+    __syncthreads();
+  }
 
   // ---- 2. D1 by wavefront 0; one panel behind it, waves 1-2 form that
   //         panel's L21 and its rank-16 update of A21 and D2, wave 3 the
@@ -483,7 +578,16 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     else                                             \
       l21_updates<Q, 1>(Ls, Xq, d2, li, lg);         \
   }
-  if (wave == 0) factor64_step<16, P, 0>(Ls, 0, lane, bad1);
+  if (wave == 0) {
+    factor64_step<16, P, 0>(Ls, 0, lane, bad1);
+  } else if constexpr (PRE) {  // rows 64..127: 26 tiles
+    if (wave == 1)
+      pre_tiles<4, 0, 5, 5, 0, 4>(Ls, Ph, li, lg);
+    else if (wave == 2)
+      pre_tiles<5, 4, 6, 6, 0, 7>(Ls, Ph, li, lg);
+    else
+      pre_tiles<7, 0, 8, 0, 0, 0>(Ls, Ph, li, lg);
+  }
   __syncthreads();
   if (wave == 0) {
     factor64_step<16, P, 1>(Ls, 0, lane, bad1);
@@ -552,12 +656,19 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
       }
     }
   };
+  // W is lower triangular; Ws holds it on and below the diagonal only.
   auto store_w16 = [&](int r0) {
     float4 v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int e = lane + 64 * u;
-      v[u] = *reinterpret_cast<const float4*>(&Ws[(r0 + (e >> 5)) * P + (e & 31) * 4]);
+      const int r = r0 + (e >> 5), c = (e & 31) * 4;
+      float4 w = *reinterpret_cast<const float4*>(&Ws[r * P + c]);
+      if (c > r) w.x = 0.f;
+      if (c + 1 > r) w.y = 0.f;
+      if (c + 2 > r) w.z = 0.f;
+      if (c + 3 > r) w.w = 0.f;
+      v[u] = w;
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -708,8 +819,21 @@ int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const
   }
 #define CHOL_DIAG_LAUNCH(FA, LA, BU) \
   hipLaunchKernelGGL((k_diag_factor<FA, LA, BU>), g, b, 0, s, p, k0, winv, bc0, bc1, bk0, bt, bfirst, zr0, zr1, nz)
+#define CHOL_DIAG_LAUNCH_PRE(LA)                                                                                  \
+  hipLaunchKernelGGL((k_diag_factor<false, LA, 2, true>), g, b, 0, s, p, k0, winv, bc0, bc1, bk0, bt, bfirst, zr0, \
+                     zr1, nz)
   const int bu = bulk ? (bulk->split ? 2 : 1) : 0;
-  if (from_a && last) {
+  if (bulk && bulk->pre) {
+    if (!bulk->split || from_a || bulk->c0 != k0 || !vec_ok(p)) {
+      set_error("k_diag_factor: in-kernel update needs the split form, c0 = k0 and float4 access");
+      return CHOL_E_ARG;
+    }
+    if (last) {
+      CHOL_DIAG_LAUNCH_PRE(true);
+    } else {
+      CHOL_DIAG_LAUNCH_PRE(false);
+    }
+  } else if (from_a && last) {
     CHOL_DIAG_LAUNCH(true, true, 0);
   } else if (from_a) {
     CHOL_DIAG_LAUNCH(true, false, 0);
@@ -731,6 +855,7 @@ int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const
     }
   }
 #undef CHOL_DIAG_LAUNCH
+#undef CHOL_DIAG_LAUNCH_PRE
   return launch_status("k_diag_factor");
 }
 

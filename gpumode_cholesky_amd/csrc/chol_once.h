@@ -241,11 +241,12 @@ __device__ __forceinline__ void gemm_tile_once(const float* Abase, int64_t lda, 
 
 // Tile `bid` of the one-shot SYRK on matrix b: C -= P P^T, P = the 128
 // columns from k0, C = the lower tiles of columns [col0, col1) (col1 < 0:
-// to n); first: old values of C from A.  Also run by extra workgroups of
-// the diagonal-block kernel, on its LDS.
+// to n); first: old values of C from A; skip_diag: the 128 x 128 block at
+// col0 is left out (the diagonal-block kernel updates it itself).  Also run
+// by extra workgroups of the diagonal-block kernel, on its LDS.
 template <int T, bool VEC, bool SPLIT>
 __device__ __forceinline__ void syrk_once_tile(const Batch& p, int bid, int64_t b, int col0, int col1, int k0, bool first,
-                               float* lds) {
+                               float* lds, bool skip_diag = false) {
   const Mat M = mat_at(p, b);
   const int c1 = col1 < 0 ? M.n : col1;
   if (c1 <= col0) return;
@@ -259,6 +260,7 @@ __device__ __forceinline__ void syrk_once_tile(const Batch& p, int bid, int64_t 
   const float* Cs = first ? M.A : M.L;
   const int i0 = col0 + T * ti;
   const int j0 = col0 + T * tj;
+  if (skip_diag && i0 < col0 + 128 && j0 < col0 + 128) return;
   gemm_tile_once<T, T, 0, VEC, SPLIT>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, Lb, Cs, M.ld, i0, j0, M.n, c1,
                                       lds);
 }
