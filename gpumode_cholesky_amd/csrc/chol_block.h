@@ -64,6 +64,19 @@ __device__ __forceinline__ void factor64_step(float* Ls, int d, int lane, int& b
           c = mma_nt_p<P>(&Ls[(d + 16 * ti) * P + c0 + kk], &Ls[(d + 16 * tj) * P + c0 + kk], c, li, lg);
         acc[nt++] = c;
       }
+    // all old values first, then every element written back (a diagonal
+    // tile's upper part unchanged): no branch, one LDS round trip.  The
+    // block is this wavefront's alone while it factors it.
+    float old[10][4];
+    nt = 0;
+#pragma unroll
+    for (int ti = t0; ti < 4; ++ti)
+#pragma unroll
+      for (int tj = t0; tj <= ti; ++tj) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) old[nt][v] = Ls[(d + 16 * ti + 4 * lg + v) * P + d + 16 * tj + li];
+        ++nt;
+      }
     nt = 0;
 #pragma unroll
     for (int ti = t0; ti < 4; ++ti)
@@ -72,7 +85,7 @@ __device__ __forceinline__ void factor64_step(float* Ls, int d, int lane, int& b
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
-          if (ti != tj || j <= i) Ls[(d + i) * P + d + j] -= acc[nt][v];
+          Ls[(d + i) * P + d + j] = (ti != tj || j <= i) ? old[nt][v] - acc[nt][v] : old[nt][v];
         }
         ++nt;
       }
@@ -136,9 +149,11 @@ __device__ __forceinline__ int factor32_pair(float* st, float* cb, int lane) {
   for (int q = 0; q < 2; ++q) {
     float* t = st + (32 * q + 16) * P;
     const floatx4_b c = mma_nt_p<P>(t, t, floatx4_b{0.f, 0.f, 0.f, 0.f}, li, lg);
+    float o[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-      if (li <= 4 * lg + v) t[(4 * lg + v) * P + 16 + li] -= c[v];
+    for (int v = 0; v < 4; ++v) o[v] = t[(4 * lg + v) * P + 16 + li];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) t[(4 * lg + v) * P + 16 + li] = (li <= 4 * lg + v) ? o[v] - c[v] : o[v];
   }
   wave_lds_sync();
   {

@@ -257,9 +257,9 @@ __device__ __forceinline__ void l21_updates(float* Ls, const float* Xq, floatx4 
       const int ti = u >= 6 ? 3 : u >= 3 ? 2 : u >= 1 ? 1 : 0;
       const int tj = u - ti * (ti + 1) / 2;
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (ti != tj || li <= 4 * lg + v)  // diagonal tile: only j <= i
-          Ls[(H + 16 * ti + 4 * lg + v) * P + H + 16 * tj + li] = old[k][v] - d2[k][v];
+      for (int v = 0; v < 4; ++v)  // diagonal tile: only j <= i changes
+        Ls[(H + 16 * ti + 4 * lg + v) * P + H + 16 * tj + li] =
+            (ti != tj || li <= 4 * lg + v) ? old[k][v] - d2[k][v] : old[k][v];
     }
   }
 }
@@ -405,8 +405,8 @@ __device__ __forceinline__ void pre_tiles(float* Ls, const __bf16* Ph, int li, i
   for (int t = 0; t < NT; ++t) {
     const int ti = t < NA ? TA : TB, tj = t < NA ? JA0 + t : JB0 + t - NA;
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
-      if (ti != tj || li <= 4 * lg + v) Ls[(16 * ti + 4 * lg + v) * P + 16 * tj + li] = old[t][v] - acc[t][v];
+    for (int v = 0; v < 4; ++v)  // diagonal tile: only j <= i changes
+      Ls[(16 * ti + 4 * lg + v) * P + 16 * tj + li] = (ti != tj || li <= 4 * lg + v) ? old[t][v] - acc[t][v] : old[t][v];
   }
 }
 
