@@ -35,7 +35,7 @@ from gpumode_cholesky_amd import harness
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-PMC_TRAFFIC_4096x32 = (8261.1 * 2.0 + 16384.0) * 1024  # bytes per launch
+PMC_TRAFFIC_4096x32 = (8260.1 * 2.0 + 16384.0) * 1024  # bytes per launch
 
 
 def _parse():
