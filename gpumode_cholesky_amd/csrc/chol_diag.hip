@@ -137,10 +137,10 @@ __device__ __forceinline__ void w_diag_inv(const float* Ls, float* Ws, int d0, i
 #pragma unroll
     for (int i = q + 1; i < 16; ++i) w[i] = fmaf(-L[i][q], w[q], w[i]);
   }
-  if (lane < 16) {
+  // every lane group holds the same columns: all store (same values), so
+  // the code stays one basic block and interleaves with the caller's MFMAs
 #pragma unroll
-    for (int i = 0; i < 16; ++i) Ws[(d0 + i) * P + d0 + c] = w[i];
-  }
+  for (int i = 0; i < 16; ++i) Ws[(d0 + i) * P + d0 + c] = w[i];
 }
 
 // W_Rm = -W_RR T[m] for m in [M0, M1) (W_RR already in Ws).
