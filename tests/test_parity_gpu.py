@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 import torch
 
-from gpumode_cholesky_amd import cholesky, custom_kernel
+from gpumode_cholesky_amd import _lib, cholesky, custom_kernel
 from gpumode_cholesky_amd import harness
 
 pytestmark = pytest.mark.gpu
@@ -210,6 +210,37 @@ def test_unaligned_rows_wide_updates(oracle_mod, mode):
     _say(f"[unaligned-wide] {batch}x{n} {mode}: resid={fro.max():.3e} {msg}")
     assert fro.max() <= (MIXED_RESID if mode == "mixed" else RESID_C * math.sqrt(n) * EPS32)
     assert (np.triu(Lc, 1) == 0).all()
+
+
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+@pytest.mark.parametrize("n,ld", [(300, 304), (640, 644), (640, 641)])
+def test_padded_leading_dimension(oracle_mod, mode, n, ld):
+    # Through the C ABI with rows padded to ld: a multiple of 4 keeps the
+    # float4 paths (and, mixed, the next block's update inside the
+    # diagonal-block kernel), an odd ld takes the scalar paths.  The padding
+    # of A is NaN (never read) and that of L a sentinel (never written).
+    batch = 3
+    a = harness.generate_input(batch, n, 2, 5100 + n, device="cpu")
+    A = torch.full((batch, n, ld), float("nan"))
+    A[:, :, :n] = a
+    L = torch.full((batch, n, ld), -7.0)
+    A, L = A.to(DEV), L.to(DEV)
+    lib = _lib.load()
+    m = 1 if mode == "mixed" else 0
+    ws_n = lib.chol_workspace_bytes(batch, n, m)
+    ws = torch.empty((max(ws_n, 1),), dtype=torch.uint8, device=DEV)
+    _say(f"[padded-ld] start {batch}x{n} ld={ld} {mode}")
+    rc = lib.chol_potrf_f32(A.data_ptr(), L.data_ptr(), batch, n, n * ld, ld, m, None, ws.data_ptr(), ws_n,
+                            torch.cuda.current_stream(DEV).cuda_stream)
+    torch.cuda.synchronize()
+    assert rc == 0, lib.chol_last_error()
+    Lc = L.cpu()
+    assert (Lc[:, :, n:] == -7.0).all()
+    Lf = Lc[:, :, :n].contiguous().numpy()
+    fro, _, _ = oracle_mod.residual(a.numpy(), Lf)
+    _say(f"[padded-ld] {batch}x{n} ld={ld} {mode}: resid={fro.max():.3e}")
+    assert fro.max() <= (MIXED_RESID if mode == "mixed" else RESID_C * math.sqrt(n) * EPS32)
+    assert (np.triu(Lf, 1) == 0).all()
 
 
 @pytest.mark.parametrize("spec", [s for s in harness.BENCH_SPECS if s["n"] > 256],
