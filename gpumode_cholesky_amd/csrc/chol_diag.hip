@@ -65,6 +65,9 @@ struct __align__(16) DiagLds {
 // PRE: a previous-panel block as BF16 hi / lo fills Ws and runs into Xp,
 // which is unused until then.
 static_assert(2 * NB * PH * 2 <= (NB * P + 2 * H * XP) * 4, "staged BF16 block exceeds Ws + Xp");
+static_assert(once_lds_floats<64, 128, true>() * 4 <= sizeof(DiagLds) &&
+                  once_lds_floats<64, 128, false>() * 4 <= sizeof(DiagLds),
+              "bulk update tile exceeds the kernel's LDS");
 
 // C(16x16) += X Y^T, rows of X (pitch px) and Y (pitch py) read as float4, K = 16.
 __device__ __forceinline__ floatx4 mma_nt2(const float* X, int px, const float* Y, int py, floatx4 acc, int li,
@@ -432,8 +435,8 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     }
     id -= nz;
     if constexpr (BULK != 0) {
-      syrk_once_tile<64, true, BULK == 2>(p, id % bt, id / bt, bc0, bc1, bk0, bfirst != 0,
-                                          reinterpret_cast<float*>(&sm), PRE);
+      syrk_once_rect<64, 128, true, BULK == 2>(p, id % bt, id / bt, bc0, bc1, bk0, bfirst != 0,
+                                               reinterpret_cast<float*>(&sm), PRE);
     }
     return;
   }
@@ -785,9 +788,7 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
 }  // namespace
 
 int64_t diag_bulk_tiles(const Batch& p, const DiagBulk& u) {
-  const int TI = (p.n - u.c0 + 63) / 64;
-  const int TJ = (u.c1 - u.c0 + 63) / 64;
-  return ((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2) * p.batch;
+  return (int64_t)rect_tiles<64, 128>(p.n, u.c0, u.c1) * p.batch;
 }
 
 int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const DiagBulk* bulk,
@@ -808,9 +809,7 @@ int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const
   }
   int bc0 = 0, bc1 = 0, bk0 = 0, bt = 1, bfirst = 0;
   if (bulk) {  // dense, float4-aligned (vec_ok), not the first block
-    const int TI = (p.n - bulk->c0 + 63) / 64;
-    const int TJ = (bulk->c1 - bulk->c0 + 63) / 64;
-    bt = (int)((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2);
+    bt = rect_tiles<64, 128>(p.n, bulk->c0, bulk->c1);
     bc0 = bulk->c0;
     bc1 = bulk->c1;
     bk0 = bulk->k0;

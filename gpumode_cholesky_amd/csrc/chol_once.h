@@ -241,12 +241,10 @@ __device__ __forceinline__ void gemm_tile_once(const float* Abase, int64_t lda, 
 
 // Tile `bid` of the one-shot SYRK on matrix b: C -= P P^T, P = the 128
 // columns from k0, C = the lower tiles of columns [col0, col1) (col1 < 0:
-// to n); first: old values of C from A; skip_diag: the 128 x 128 block at
-// col0 is left out (the diagonal-block kernel updates it itself).  Also run
-// by extra workgroups of the diagonal-block kernel, on its LDS.
+// to n); first: old values of C from A.
 template <int T, bool VEC, bool SPLIT>
 __device__ __forceinline__ void syrk_once_tile(const Batch& p, int bid, int64_t b, int col0, int col1, int k0, bool first,
-                               float* lds, bool skip_diag = false) {
+                               float* lds) {
   const Mat M = mat_at(p, b);
   const int c1 = col1 < 0 ? M.n : col1;
   if (c1 <= col0) return;
@@ -260,9 +258,47 @@ __device__ __forceinline__ void syrk_once_tile(const Batch& p, int bid, int64_t 
   const float* Cs = first ? M.A : M.L;
   const int i0 = col0 + T * ti;
   const int j0 = col0 + T * tj;
-  if (skip_diag && i0 < col0 + 128 && j0 < col0 + 128) return;
   gemm_tile_once<T, T, 0, VEC, SPLIT>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, Lb, Cs, M.ld, i0, j0, M.n, c1,
                                       lds);
+}
+
+// Rectangular one-shot tiles of the same update, TM rows x TN columns
+// (TN = 2 TM): column tile t covers columns col0 + TN t .. + TN and the
+// rows from its diagonal to n in TM-row tiles.  A workgroup stages its
+// TN-row operand once for two TM x TM tiles' worth of work, so the extra
+// workgroups of a diagonal-block launch take half as many rounds of the
+// idle CUs (on its LDS).  rect_tiles() counts them per matrix; skip_diag:
+// the 128 x 128 block at col0 is left out (the diagonal-block kernel
+// updates it itself).
+template <int TM, int TN>
+__host__ __device__ __forceinline__ int rect_tiles(int n, int col0, int col1) {
+  const int TJ = (col1 - col0 + TN - 1) / TN;
+  int total = 0;
+  for (int t = 0; t < TJ; ++t) total += (n - col0 - TN * t + TM - 1) / TM;
+  return total;
+}
+
+template <int TM, int TN, bool VEC, bool SPLIT>
+__device__ __forceinline__ void syrk_once_rect(const Batch& p, int bid, int64_t b, int col0, int col1, int k0, bool first,
+                                               float* lds, bool skip_diag) {
+  const Mat M = mat_at(p, b);
+  const int c1 = col1 < 0 ? M.n : col1;
+  if (c1 <= col0) return;
+  const int TJ = (c1 - col0 + TN - 1) / TN;
+  int t = 0, rem = bid;
+  for (; t < TJ; ++t) {
+    const int R = (M.n - col0 - TN * t + TM - 1) / TM;
+    if (rem < R) break;
+    rem -= R;
+  }
+  if (t >= TJ) return;
+  const int i0 = col0 + TN * t + TM * rem;
+  const int j0 = col0 + TN * t;
+  if (skip_diag && i0 < col0 + 128 && j0 < col0 + 128) return;
+  float* Lb = M.L;
+  const float* Cs = first ? M.A : M.L;
+  gemm_tile_once<TM, TN, 0, VEC, SPLIT>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, Lb, Cs, M.ld, i0, j0, M.n, c1,
+                                        lds);
 }
 }  // namespace
 }  // namespace chol
