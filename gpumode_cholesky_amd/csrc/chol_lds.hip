@@ -165,10 +165,21 @@ __device__ __forceinline__ void lds_factor(const Batch& p, int64_t b) {
         d = __builtin_amdgcn_mfma_f32_16x16x4f32(af.z, bf.z, d, 0, 0, 0);
         d = __builtin_amdgcn_mfma_f32_16x16x4f32(af.w, bf.w, d, 0, 0, 0);
         const int j = j0 + li;
+        if (I != J) {
+          // below the diagonal tile: every element, old values loaded together
+          float o[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = i0 + 4 * lg + v;
-          if (I != J || j <= i) S[row_off(i) + j] -= d[v];
+          for (int v = 0; v < 4; ++v) o[v] = S[row_off(i0 + 4 * lg + v) + j];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) S[row_off(i0 + 4 * lg + v) + j] = o[v] - d[v];
+        } else {
+          // diagonal tile: only j <= i (in the packed rows, a row's
+          // elements past its diagonal group belong to the next rows)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = i0 + 4 * lg + v;
+            if (j <= i) S[row_off(i) + j] -= d[v];
+          }
         }
       }
     }
