@@ -106,7 +106,7 @@ __device__ __forceinline__ void gemm_tile(const float* Abase, int64_t lda, int a
 
   float4 ra[PA], rb[PB];
   load_rows<BM, VEC>(ra, Abase, lda, a_row0, a_lim, ka, tid);
-  load_rows<BN, VEC>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
+  load_rows<BN, VEC, KC, MODE == 1>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
   store_rows<BM>(As[0], ra, tid);
   store_rows<BN>(Bs[0], rb, tid);
   __syncthreads();
@@ -117,7 +117,7 @@ __device__ __forceinline__ void gemm_tile(const float* Abase, int64_t lda, int a
     const bool more = ch + 1 < nchunks;
     if (more) {
       load_rows<BM, VEC>(ra, Abase, lda, a_row0, a_lim, ka + (ch + 1) * KC, tid);
-      load_rows<BN, VEC>(rb, Bbase, ldb, b_row0, b_lim, kb + (ch + 1) * KC, tid);
+      load_rows<BN, VEC, KC, MODE == 1>(rb, Bbase, ldb, b_row0, b_lim, kb + (ch + 1) * KC, tid);
     }
     const float* as = As[cur];
     const float* bs = Bs[cur];
@@ -199,7 +199,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda
 
   float4 ra[PA], rb[PB];
   load_rows<BM, VEC>(ra, Abase, lda, a_row0, a_lim, ka, tid);
-  load_rows<BN, VEC>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
+  load_rows<BN, VEC, KC, MODE == 1>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
   store_rows_split<BM>(Ah[0], Al[0], ra, tid);
   store_rows_split<BN>(Bh[0], Bl[0], rb, tid);
   __syncthreads();
@@ -210,7 +210,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda
     const bool more = ch + 1 < nchunks;
     if (more) {
       load_rows<BM, VEC>(ra, Abase, lda, a_row0, a_lim, ka + (ch + 1) * KC, tid);
-      load_rows<BN, VEC>(rb, Bbase, ldb, b_row0, b_lim, kb + (ch + 1) * KC, tid);
+      load_rows<BN, VEC, KC, MODE == 1>(rb, Bbase, ldb, b_row0, b_lim, kb + (ch + 1) * KC, tid);
     }
     bf16x8 fah[FM], fal[FM], fbh[FN], fbl[FN];
 #pragma unroll

@@ -15,7 +15,10 @@ constexpr int KC = 32;        // K chunk staged in LDS
 constexpr int KP = KC + 4;    // LDS row pitch (floats)
 constexpr int kThreads = 256;
 
-template <int ROWS, bool VEC, int KW = KC>
+// LOWER: the rows are those of a lower-triangular matrix whose row and
+// column indices are the global ones (the inverse W of the panel): chunks
+// wholly right of its diagonal hold zeros and are not read.
+template <int ROWS, bool VEC, int KW = KC, bool LOWER = false>
 __device__ __forceinline__ void load_rows(float4 (&reg)[ROWS * KW / 4 / kThreads], const float* base,
                                           int64_t ld, int row0, int rlim, int k, int tid) {
   // ROWS x KW floats as float4 chunks; thread t takes chunks t, t+256, ...
@@ -27,7 +30,7 @@ __device__ __forceinline__ void load_rows(float4 (&reg)[ROWS * KW / 4 / kThreads
     const int q = (c % (KW / 4)) * 4;
     const int gr = row0 + r;
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (gr < rlim) {
+    if (gr < rlim && !(LOWER && k + q > gr)) {
       const float* src = base + (int64_t)gr * ld + k + q;
       if constexpr (VEC) {
         v = gld4(src);
@@ -123,7 +126,7 @@ __device__ __forceinline__ void gemm_tile_once(const float* Abase, int64_t lda, 
 
   float4 ra[PA], rb[PB];
   load_rows<BM, VEC, K1>(ra, Abase, lda, a_row0, a_lim, ka, tid);
-  load_rows<BN, VEC, K1>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
+  load_rows<BN, VEC, K1, MODE == 1>(rb, Bbase, ldb, b_row0, b_lim, kb, tid);
 
   // C tile of this wave: offsets, masks and (MODE 0) the old values, whose
   // loads fly together with the operand loads.  A masked element reads
