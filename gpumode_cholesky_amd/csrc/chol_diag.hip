@@ -745,9 +745,9 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   }
   if constexpr (!LAST) __syncthreads();
 
-  // ---- 4. the tail: wave 0 stores D2's factor and forms the sums Q_m of
-  //         W2's block row 3, wave 3 its diagonal block W2_33, waves 1-2
-  //         W21's block row 2 (and U); then W2_3m = -W2_33 Q_m (waves 0, 3)
+  // ---- 4. the tail: waves 0-1 store D2's factor, wave 0 forms the sums
+  //         Q_m of W2's block row 3, wave 3 its diagonal block W2_33, waves
+  //         1-2 W21's block row 2 (and U); then W2_3m = -W2_33 Q_m (waves 0, 3)
   //         and W21's block row 3 = -W2_33 U (waves 1-2); then the stores
   //         of W rows 64..127. ----
   if (wave == 0) {
@@ -755,14 +755,16 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
     if (p.info && lane == 0 && bad != 0 && bad <= nb && p.info[M.slot] == 0) p.info[M.slot] = k0 + bad;
     if constexpr (!LAST) w_row_sums<3>(Ls, Ws, Ts, H, li, lg);
     store_l32(H, H);
-    store_l32(H + 32, H);
+    if constexpr (LAST) store_l32(H + 32, H);
   } else if constexpr (!LAST) {
-    if (wave == 1)
+    if (wave == 1) {
       w21_tiles<2, 0, 2>(Ls, Ws, U, li, lg);
-    else if (wave == 2)
+      store_l32(H + 32, H);  // D2's lower half (wave 0 has the longer round)
+    } else if (wave == 2) {
       w21_tiles<2, 1, 3>(Ls, Ws, U, li, lg);
-    else
+    } else {
       w_diag_inv(Ls, Ws, H + 48, lane, li);
+    }
   }
   if constexpr (!LAST) {
     __syncthreads();
