@@ -38,7 +38,7 @@ static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 constexpr int64_t kLdsMaxN = 256;
 
 // Tuning table (chol_set_tuning): rows {n_lo, n_hi, batch_lo, batch_hi, nbo}.
-// nbo > 0: blocked path with that outer block width (a multiple of 64);
+// nbo > 0: blocked path with that outer block width (a multiple of kNB = 128);
 // nbo = -1: the whole-matrix LDS kernel (n <= 256 only).  First match wins.
 struct TuneRow {
   int64_t n_lo, n_hi, b_lo, b_hi, nbo;

@@ -82,7 +82,7 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n,
 /*
  * Route / blocking table written by the autotuner: `rows` rows of 5 int64
  *   {n_lo, n_hi, batch_lo, batch_hi, nbo}   (inclusive ranges)
- * nbo = outer block width of the blocked path (a multiple of 64, <= 4096),
+ * nbo = outer block width of the blocked path (a multiple of 128, <= 4096),
  * or -1 to keep the whole-matrix LDS kernel (only for n_hi <= 256).  The
  * first matching row wins; unmatched shapes use built-in defaults; rows = 0
  * clears the table.  Replaces the reference's hand-written per-shape tables

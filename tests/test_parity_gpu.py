@@ -193,6 +193,30 @@ def test_mixed_against_oracle(oracle_mod, batch, n, case):
     assert (np.triu(Lc, 1) == 0).all()
 
 
+@pytest.mark.parametrize("nbo", [384, 640])
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_outer_block_widths_not_powers_of_two(oracle_mod, mode, nbo):
+    # Outer blocks of three or five 128-column steps through the tuning
+    # table, n = 1100 leaving a partial last block (and a last diagonal block
+    # of 76 columns): look-ahead bulk and narrow updates at these widths.
+    from gpumode_cholesky_amd import autotune
+    n = 1100
+    a = harness.generate_input(2, n, 3, 7300 + nbo, "dense", device="cpu")
+    try:
+        autotune.set_table([[n, n, 1, 1 << 20, nbo]])
+        L = cholesky(a.to(DEV), mode=mode)
+        torch.cuda.synchronize()
+    finally:
+        autotune.load_table()
+    ok, msg = harness.check_implementation(a.to(DEV), L)
+    assert ok, msg
+    Lc = L.cpu().numpy()
+    fro, _, _ = oracle_mod.residual(a.numpy(), Lc)
+    _say(f"[nbo] {mode} n={n} nbo={nbo}: resid={fro.max():.3e}")
+    assert fro.max() <= (MIXED_RESID if mode == "mixed" else RESID_C * math.sqrt(n) * EPS32)
+    assert (np.triu(Lc, 1) == 0).all()
+
+
 @pytest.mark.parametrize("mode", ["plain", "mixed"])
 def test_unaligned_rows_wide_updates(oracle_mod, mode):
     # n = 1001 (rows not 16-byte aligned) and 96 matrices: the outer-block
