@@ -201,33 +201,7 @@ __global__ __launch_bounds__(kThreads) void k_potrf_lds(Batch p) {
   lds_factor<N>(p, blockIdx.x);
 }
 
-// Persistent form for ragged batches: workgroups pop matrices of the sorted
-// range [first, first + count) from a work queue (an atomic counter) and
-// leave when it is exhausted.
-template <int N>
-__global__ __launch_bounds__(kThreads) void k_potrf_lds_queue(Batch p, int64_t first, int64_t count, int* queue) {
-  __shared__ int slot_s;
-  for (;;) {
-    if (threadIdx.x == 0) slot_s = atomicAdd(queue, 1);
-    __syncthreads();
-    const int slot = slot_s;
-    __syncthreads();
-    if (slot >= count) break;
-    lds_factor<N>(p, first + slot);
-    __syncthreads();
-  }
-}
-
 }  // namespace
-
-int launch_lds_queue(const Batch& p, int64_t first, int64_t count, int nmax, int* queue, int grid, hipStream_t s) {
-  if (count <= 0) return 0;
-  if (nmax <= 128)
-    hipLaunchKernelGGL(k_potrf_lds_queue<128>, dim3((unsigned)grid), dim3(kThreads), 0, s, p, first, count, queue);
-  else
-    hipLaunchKernelGGL(k_potrf_lds_queue<256>, dim3((unsigned)grid), dim3(kThreads), 0, s, p, first, count, queue);
-  return launch_status("k_potrf_lds_queue");
-}
 
 int launch_lds(const Batch& p, hipStream_t s) {
   if (p.n <= 128)

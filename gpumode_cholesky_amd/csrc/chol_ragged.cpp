@@ -3,17 +3,20 @@
 //
 // The host sorts the matrices by order (largest first) and uploads one
 // descriptor table (pointers, orders, original positions) into the
-// workspace.  The sorted list splits into three size classes, each one
+// workspace.  The sorted list splits into two size classes, each one
 // launch sequence over its contiguous slice:
-//   n <= 64       : persistent wavefront kernel (k_potrf_small_queue), one
-//                   launch per padded order 64/32/16, fed by an atomic work
-//                   queue over the sorted slice;
-//   64 < n <= 256 : persistent workgroup kernel with the matrix in LDS
-//                   (k_potrf_lds_queue), one launch per padded order 256/128;
-//   n > 256       : the blocked path run in lock step over 64-column steps:
-//                   step k0 touches the matrices with n > k0, a prefix of the
-//                   sorted list, and each kernel reads its matrix's order
-//                   from the table (blocks past a smaller matrix exit).
+//   n <= 64 : persistent wavefront kernel (k_potrf_small_queue), one
+//             launch per padded order 64/32/16, fed by an atomic work queue
+//             over the sorted slice;
+//   n > 64  : the blocked path run in lock step over 128-column steps: step
+//             k0 touches the matrices with n > k0, a prefix of the sorted
+//             list, and each kernel reads its matrix's order from the table
+//             (blocks past a smaller matrix exit).  Matrices of order
+//             65..256 join only the first one or two diagonal-block
+//             launches, whose workgroup count (one per CU) already takes
+//             two rounds when the larger class has more than 256 matrices;
+//             a whole-matrix LDS kernel per matrix measured ~3x slower per
+//             matrix at n = 128 and 256.
 // The reference factors one fixed (batch, n) per call (custom_kernel,
 // submission.py:4101-4268); this generalises its per-shape dispatch.
 #include <algorithm>
@@ -35,7 +38,7 @@ struct Layout {
 Layout layout(const int64_t* n, int64_t count) {
   Layout l{};
   int64_t nbig = 0;
-  for (int64_t i = 0; i < count; ++i) nbig += n[i] > 256;
+  for (int64_t i = 0; i < count; ++i) nbig += n[i] > 64;
   l.queues = 0;
   l.Ap = align256(l.queues + 8 * sizeof(int));
   l.Lp = align256(l.Ap + count * sizeof(void*));
@@ -50,7 +53,6 @@ Layout layout(const int64_t* n, int64_t count) {
 }  // namespace
 
 int launch_small_queue(const Batch& p, int64_t first, int64_t count, int np, int* queue, int grid, hipStream_t s);
-int launch_lds_queue(const Batch& p, int64_t first, int64_t count, int nmax, int* queue, int grid, hipStream_t s);
 
 }  // namespace chol
 
@@ -136,20 +138,19 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
     while (i < count && ns[i] > lim) ++i;
     return i;
   };
-  const int64_t c256 = first_at_most(256), c128 = first_at_most(128), c64 = first_at_most(64),
-                c32 = first_at_most(32), c16 = first_at_most(16), cz = first_at_most(0);
+  const int64_t c64 = first_at_most(64), c32 = first_at_most(32), c16 = first_at_most(16), cz = first_at_most(0);
 
-  // n > 256: lock-step blocked path over the prefix [0, c256).
-  if (c256 > 0) {
+  // n > 64: lock-step blocked path over the prefix [0, c64).
+  if (c64 > 0) {
     Batch q = base;
-    q.batch = c256;
+    q.batch = c64;
     q.n = ns[0];
     float* winv = reinterpret_cast<float*>(w + lay.winv);
     if ((rc = launch_copy_lower(q, s))) return rc;
     for (int k0 = 0; k0 < ns[0]; k0 += kNB) {
       int64_t act = 0, act2 = 0;
-      while (act < c256 && ns[act] > k0) ++act;
-      while (act2 < c256 && ns[act2] > k0 + kNB) ++act2;
+      while (act < c64 && ns[act] > k0) ++act;
+      while (act2 < c64 && ns[act2] > k0 + kNB) ++act2;
       q.batch = act;
       q.n = ns[0];
       if ((rc = launch_diag_factor(q, k0, winv, s))) return rc;
@@ -159,11 +160,6 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
       if ((rc = launch_syrk(q, k0 + kNB, -1, k0, kNB, mode == CHOL_MODE_MIXED, s))) return rc;
     }
   }
-  // 64 < n <= 256: persistent LDS workgroups.
-  if ((rc = launch_lds_queue(base, c256, c128 - c256, 256, queues + 0, (int)std::min<int64_t>(c128 - c256, 256), s)))
-    return rc;
-  if ((rc = launch_lds_queue(base, c128, c64 - c128, 128, queues + 1, (int)std::min<int64_t>(c64 - c128, 1024), s)))
-    return rc;
   // n <= 64: persistent wavefronts (4 per workgroup, 64/np matrices each).
   auto waves = [](int64_t cnt, int np) { return (cnt * np + 63) / 64; };
   if ((rc = launch_small_queue(base, c64, c32 - c64, 64, queues + 2,

@@ -3,7 +3,7 @@
 ``cholesky_ragged(mats)`` factors a list of FP32 SPD device tensors of mixed
 orders through ``chol_potrf_ragged_f32``: the library sorts them by order and
 runs each size class as one launch sequence (persistent work-queue kernels
-for n <= 256, the blocked path in lock step above).  The reference has no
+for n <= 64, the blocked path in lock step above).  The reference has no
 ragged entry point (it factors one fixed (batch, n) per call,
 submission.py:4101-4268).
 """

@@ -98,9 +98,8 @@ int chol_set_tuning(const int64_t* table, int64_t rows);
  * matrix i; the three arrays themselves are in HOST memory.  info: nullable
  * device int32[count], in the caller's order.  The library sorts by order
  * and runs each size class as one launch sequence: a persistent wavefront
- * kernel (n <= 64) and a persistent LDS-workgroup kernel (64 < n <= 256),
- * both fed by atomic work queues over the size-sorted list, and the blocked
- * path in lock step over the larger matrices.
+ * kernel (n <= 64) fed by atomic work queues over the size-sorted list, and
+ * the blocked path in lock step over the matrices with n > 64.
  * Has no counterpart in the reference, which factors one fixed (batch, n)
  * per call; it generalises the per-shape dispatch of submission.py:4101-4268.
  */

@@ -37,6 +37,21 @@ def test_ragged_mixed_orders_against_oracle(oracle_mod):
             assert torch.equal(dense, L), f"matrix {i} n={n}: differs from the dense path"
 
 
+@pytest.mark.parametrize("sizes", [[65, 100, 128, 77, 128], [129, 200, 256, 65, 100, 150, 255, 33]])
+def test_ragged_mid_orders_only(oracle_mod, sizes):
+    # Orders 65..256 run the lock-step blocked path; with no larger matrix,
+    # the block that ends the largest is every matrix's last (no inverse).
+    mats = [harness.generate_input(1, n, 3, 900 + i, device="cpu")[0] for i, n in enumerate(sizes)]
+    outs, info = cholesky_ragged([m.to(DEV) for m in mats], return_info=True)
+    assert (info.cpu() == 0).all()
+    for i, (a, L) in enumerate(zip(mats, outs)):
+        n = a.shape[0]
+        Lc = L.cpu().numpy()
+        assert (np.triu(Lc, 1) == 0).all(), f"matrix {i} n={n}: non-zero above the diagonal"
+        fro = oracle_mod.residual(a.numpy()[None], Lc[None])[0][0]
+        assert fro <= 8 * math.sqrt(n) * EPS32, f"matrix {i} n={n}: residual {fro:.3e}"
+
+
 def test_ragged_info_in_caller_order():
     sizes = [300, 40, 7, 150, 64]
     mats = [harness.generate_input(1, n, 2, 50 + n, device="cpu")[0] for n in sizes]
