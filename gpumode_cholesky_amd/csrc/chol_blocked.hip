@@ -29,39 +29,42 @@ namespace chol {
 namespace {
 
 // ---------------------------------------------------------------- copy ----
-// L = tril(A) with exact zeros above the diagonal.
+// L = tril(A) with exact zeros above the diagonal.  Workgroup x takes rows
+// x, x + gridDim.x, ... of matrix blockIdx.y (any matrix of a ragged
+// table); its threads run along the row.  Where the source and destination
+// rows share their 16-byte alignment (both buffers 16-byte aligned and any
+// ld, or equal offsets), the row moves in float4 chunks between a scalar
+// head and tail of at most 3 elements each.
 __global__ __launch_bounds__(256) void k_copy_lower(Batch p) {
   const Mat M = mat_at(p, blockIdx.y);
   const int n = M.n;
-  const float* src = M.A;
-  float* dst = M.L;
-  const int64_t total = (int64_t)n * n;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e / n);
-    const int j = (int)(e - (int64_t)i * n);
-    gst(dst + (int64_t)i * M.ld + j, (j <= i) ? gld_nt(src + (int64_t)i * M.ld + j) : 0.f);
-  }
-}
-
-// Vector form for n % 4 == 0 and 16-byte aligned rows.
-__global__ __launch_bounds__(256) void k_copy_lower4(Batch p) {
-  const int n = p.n;
-  const int n4 = n / 4;
-  const int64_t b = blockIdx.y;
-  const float* src = p.A + b * p.bs;
-  float* dst = p.L + b * p.bs;
-  const int64_t total = (int64_t)n * n4;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e / n4);
-    const int j = 4 * (int)(e - (int64_t)i * n4);
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (j <= i) v = gld4_nt(src + (int64_t)i * p.ld + j);
-    if (j + 1 > i) v.y = 0.f;
-    if (j + 2 > i) v.z = 0.f;
-    if (j + 3 > i) v.w = 0.f;
-    gst4(dst + (int64_t)i * p.ld + j, v);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const float* src = M.A + (int64_t)i * M.ld;
+    float* dst = M.L + (int64_t)i * M.ld;
+    const int head = (int)((4 - ((reinterpret_cast<uintptr_t>(src) >> 2) & 3)) & 3);
+    const bool vec = ((reinterpret_cast<uintptr_t>(src) ^ reinterpret_cast<uintptr_t>(dst)) & 15) == 0 && head < n;
+    if (vec) {
+      const int n4 = (n - head) >> 2;
+      const int t0 = head + 4 * n4;  // first element of the tail
+      const int t = threadIdx.x;
+      int j = -1;
+      if (t < head)
+        j = t;
+      else if (t >= 64 && t - 64 < n - t0)
+        j = t0 + t - 64;
+      if (j >= 0) gst(dst + j, (j <= i) ? gld_nt(src + j) : 0.f);
+      for (int c = t; c < n4; c += blockDim.x) {
+        const int jj = head + 4 * c;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (jj <= i) v = gld4_nt(src + jj);
+        if (jj + 1 > i) v.y = 0.f;
+        if (jj + 2 > i) v.z = 0.f;
+        if (jj + 3 > i) v.w = 0.f;
+        gst4(dst + jj, v);
+      }
+    } else {
+      for (int jj = threadIdx.x; jj < n; jj += blockDim.x) gst(dst + jj, (jj <= i) ? gld_nt(src + jj) : 0.f);
+    }
   }
 }
 
@@ -104,20 +107,21 @@ int launch_zero_info(int32_t* info, int64_t batch, hipStream_t s) {
   return launch_status("k_zero_info");
 }
 
-int launch_copy_lower(const Batch& p, hipStream_t s) {
-  const bool vec = !p.Ap && ((reinterpret_cast<uintptr_t>(p.A) | reinterpret_cast<uintptr_t>(p.L)) & 15) == 0 &&
-                   (p.ld % 4) == 0 && (p.bs % 4) == 0 && (p.n % 4) == 0;
-  const int64_t elems = vec ? (int64_t)p.n * (p.n / 4) : (int64_t)p.n * p.n;
-  int64_t gx = (elems + 255) / 256;
-  if (gx > 4096) gx = 4096;
-  if (vec)
-    hipLaunchKernelGGL(k_copy_lower4, dim3((unsigned)gx, (unsigned)p.batch), dim3(256), 0, s, p);
-  else
-    hipLaunchKernelGGL(k_copy_lower, dim3((unsigned)gx, (unsigned)p.batch), dim3(256), 0, s, p);
-  return launch_status("k_copy_lower");
+// Rows per matrix for k_copy_lower: about 8 KiB of one matrix's traffic
+// per workgroup, so that a workgroup does several rows of a large matrix
+// (grid.x is set by the largest matrix of a ragged table; rows past a
+// smaller one exit).
+static unsigned row_groups(int64_t n) {
+  int64_t gx = n * n / 8192;
+  if (gx < 1) gx = 1;
+  if (gx > n) gx = n;
+  return (unsigned)gx;
 }
 
-
+int launch_copy_lower(const Batch& p, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_lower, dim3(row_groups(p.n), (unsigned)p.batch), dim3(256), 0, s, p);
+  return launch_status("k_copy_lower");
+}
 
 int launch_zero_upper(const Batch& p, hipStream_t s) {
   const bool vec = !p.Ap && (reinterpret_cast<uintptr_t>(p.L) & 15) == 0 && (p.ld % 4) == 0 &&
