@@ -20,7 +20,9 @@ def cholesky_ragged(mats, mode: str = "plain", return_info: bool = False):
     """Factor each tensor of ``mats`` ([n_i, n_i] FP32, one device).
 
     Returns the list of factors (and an int32 ``info`` tensor, one entry per
-    matrix, when ``return_info``)."""
+    matrix, when ``return_info``).  The factors are views of one device
+    buffer, each at a 256-byte aligned offset: writing one leaves the others
+    unchanged, and the buffer is freed when the last of them is."""
     m = _MODES.get(mode)
     if m is None:
         raise ValueError(f"mode must be one of {sorted(_MODES)}, got {mode!r}")
@@ -32,11 +34,17 @@ def cholesky_ragged(mats, mode: str = "plain", return_info: bool = False):
         _validate(a)
         if a.dim() != 2 or a.device != dev:
             raise ValueError("every matrix must be a 2-D [n, n] tensor on one device")
-    outs = [torch.empty_like(a) for a in mats]
     count = len(mats)
     n_host = np.array([a.shape[-1] for a in mats], dtype=np.int64)
     a_host = np.array([a.data_ptr() for a in mats], dtype=np.uint64)
-    l_host = np.array([o.data_ptr() for o in outs], dtype=np.uint64)
+    # One output allocation instead of one per matrix (the host time of
+    # hundreds of allocations is a visible share of a ragged call); the
+    # views are made after the kernels are enqueued, while they run.
+    span = (n_host * n_host + 63) // 64 * 64
+    offs = np.zeros(count, dtype=np.int64)
+    np.cumsum(span[:-1], out=offs[1:])
+    flat = torch.empty((max(int(span.sum()), 1),), dtype=torch.float32, device=dev)
+    l_host = np.uint64(flat.data_ptr()) + 4 * offs.astype(np.uint64)
     lib = _lib.load()
     nws = lib.chol_ragged_workspace_bytes(n_host.ctypes.data, count)
     ws = torch.empty((max(nws, 1),), dtype=torch.uint8, device=dev)
@@ -46,4 +54,5 @@ def cholesky_ragged(mats, mode: str = "plain", return_info: bool = False):
                                        info.data_ptr() if info is not None else None,
                                        ws.data_ptr(), nws, _raw_stream(dev.index))
     _lib.check(rc, "chol_potrf_ragged_f32")
+    outs = [flat.as_strided((n, n), (n, 1), o) for n, o in zip(n_host.tolist(), offs.tolist())]
     return (outs, info) if return_info else outs
