@@ -23,7 +23,6 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int GT = 64;       // GEMM tile (M and N)
 constexpr int GKC = 32;      // K chunk
 constexpr int GKP = GKC + 4; // LDS row pitch
 constexpr int kThreads = 256;
@@ -44,24 +43,28 @@ struct Gemm {
   int nb2;
 };
 
-template <bool KC_CONTIG>
-__device__ __forceinline__ void gload(float (&r)[GT * GKC / kThreads], const float* base, int64_t ld, int row0,
+// Loads a G x GKC tile of op(X)[row][k] into registers, G * GKC / 256
+// floats per thread.
+template <int G, bool KC_CONTIG>
+__device__ __forceinline__ void gload(float (&r)[G * GKC / kThreads], const float* base, int64_t ld, int row0,
                                       int rlim, int k0, int klim, int tid) {
-  // Loads a GT x GKC tile of op(X)[row][k] into registers, 8 floats per thread.
-  constexpr int PER = GT * GKC / kThreads;  // 8
+  constexpr int PER = G * GKC / kThreads;  // 8 (G = 64) or 16 (G = 128)
   if constexpr (KC_CONTIG) {
-    // thread: row = tid / 4, k = (tid % 4) * 8 .. +7
-    const int gr = row0 + (tid >> 2);
-    const int kk = (tid & 3) * 8;
+    // thread: row = tid / (GKC / PER), k = (tid % (GKC / PER)) * PER .. +PER-1
+    constexpr int TPR = GKC / PER;
+    const int gr = row0 + tid / TPR;
+    const int kk = (tid % TPR) * PER;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int gk = k0 + kk + u;
       r[u] = (gr < rlim && gk < klim) ? base[(int64_t)gr * ld + gk] : 0.f;
     }
   } else {
-    // thread: k = tid / 8, rows (tid % 8) * 8 .. +7 (contiguous in memory)
-    const int kk = tid >> 3;
-    const int rr = (tid & 7) * 8;
+    // thread: k = tid / (G / PER), rows (tid % (G / PER)) * PER .. +PER-1
+    // (contiguous in memory)
+    constexpr int TPK = G / PER;
+    const int kk = tid / TPK;
+    const int rr = (tid % TPK) * PER;
     const int gk = k0 + kk;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
@@ -71,90 +74,106 @@ __device__ __forceinline__ void gload(float (&r)[GT * GKC / kThreads], const flo
   }
 }
 
-template <bool KC_CONTIG>
-__device__ __forceinline__ void gstore(float* lds, const float (&r)[GT * GKC / kThreads], int tid) {
-  constexpr int PER = GT * GKC / kThreads;
+template <int G, bool KC_CONTIG>
+__device__ __forceinline__ void gstore(float* lds, const float (&r)[G * GKC / kThreads], int tid) {
+  constexpr int PER = G * GKC / kThreads;
   if constexpr (KC_CONTIG) {
-    const int rr = tid >> 2;
-    const int kk = (tid & 3) * 8;
+    constexpr int TPR = GKC / PER;
+    const int rr = tid / TPR;
+    const int kk = (tid % TPR) * PER;
 #pragma unroll
     for (int u = 0; u < PER; ++u) lds[rr * GKP + kk + u] = r[u];
   } else {
-    const int kk = tid >> 3;
-    const int rr = (tid & 7) * 8;
+    constexpr int TPK = G / PER;
+    const int kk = tid / TPK;
+    const int rr = (tid % TPK) * PER;
 #pragma unroll
     for (int u = 0; u < PER; ++u) lds[(rr + u) * GKP + kk] = r[u];
   }
 }
 
-template <bool A_KC, bool B_KC>
+// G x G output tile per workgroup of 4 wavefronts, each a (G/2) x (G/2)
+// quarter of 16x16 MFMA blocks: G = 128 for large products (four times the
+// operand reuse per LDS read of G = 64), G = 64 where 128-tiles would leave
+// CUs idle.
+template <int G, bool A_KC, bool B_KC>
 __global__ __launch_bounds__(kThreads) void k_gemm(Gemm g) {
-  __shared__ __align__(16) float As[2][GT * GKP];
-  __shared__ __align__(16) float Bs[2][GT * GKP];
+  constexpr int F = G / 32;  // 16x16 blocks per wave side
+  constexpr int PER = G * GKC / kThreads;
+  __shared__ __align__(16) float As[2][G * GKP];
+  __shared__ __align__(16) float Bs[2][G * GKP];
   const int z = blockIdx.z;
   const int z1 = z / g.nb2, z2 = z - (z / g.nb2) * g.nb2;
   const float* A = g.A + z1 * g.sA1 + z2 * g.sA2;
   const float* B = g.B + z1 * g.sB1 + z2 * g.sB2;
   float* C = g.C + z1 * g.sC1 + z2 * g.sC2;
-  const int m0 = blockIdx.y * GT;
-  const int n0 = blockIdx.x * GT;
+  const int m0 = blockIdx.y * G;
+  const int n0 = blockIdx.x * G;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = (wave >> 1) * 32;
-  const int wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * (G / 2);
+  const int wn = (wave & 1) * (G / 2);
   const int li = lane & 15;
   const int lg = lane >> 4;
 
-  floatx4 acc[2][2];
+  floatx4 acc[F][F];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < F; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < F; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  float ra[GT * GKC / kThreads], rb[GT * GKC / kThreads];
-  gload<A_KC>(ra, A, g.lda, m0, g.M, 0, g.K, tid);
-  gload<B_KC>(rb, B, g.ldb, n0, g.N, 0, g.K, tid);
-  gstore<A_KC>(As[0], ra, tid);
-  gstore<B_KC>(Bs[0], rb, tid);
+  float ra[PER], rb[PER];
+  gload<G, A_KC>(ra, A, g.lda, m0, g.M, 0, g.K, tid);
+  gload<G, B_KC>(rb, B, g.ldb, n0, g.N, 0, g.K, tid);
+  gstore<G, A_KC>(As[0], ra, tid);
+  gstore<G, B_KC>(Bs[0], rb, tid);
   __syncthreads();
   const int nch = (g.K + GKC - 1) / GKC;
   for (int ch = 0; ch < nch; ++ch) {
     const int cur = ch & 1;
     const bool more = ch + 1 < nch;
     if (more) {
-      gload<A_KC>(ra, A, g.lda, m0, g.M, (ch + 1) * GKC, g.K, tid);
-      gload<B_KC>(rb, B, g.ldb, n0, g.N, (ch + 1) * GKC, g.K, tid);
+      gload<G, A_KC>(ra, A, g.lda, m0, g.M, (ch + 1) * GKC, g.K, tid);
+      gload<G, B_KC>(rb, B, g.ldb, n0, g.N, (ch + 1) * GKC, g.K, tid);
     }
 #pragma unroll
     for (int ks = 0; ks < GKC; ks += 16) {
-      float4 fa[2], fb[2];
+      float4 fa[F], fb[F];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < F; ++a)
         fa[a] = *reinterpret_cast<const float4*>(&As[cur][(wm + a * 16 + li) * GKP + ks + 4 * lg]);
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < F; ++b)
         fb[b] = *reinterpret_cast<const float4*>(&Bs[cur][(wn + b * 16 + li) * GKP + ks + 4 * lg]);
+      // k-step outermost: consecutive MFMAs write different accumulators.
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < F; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].x, fb[b].x, acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].y, fb[b].y, acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].z, fb[b].z, acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].w, fb[b].w, acc[a][b], 0, 0, 0);
-        }
+        for (int b = 0; b < F; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].x, fb[b].x, acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < F; ++a)
+#pragma unroll
+        for (int b = 0; b < F; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].y, fb[b].y, acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < F; ++a)
+#pragma unroll
+        for (int b = 0; b < F; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].z, fb[b].z, acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < F; ++a)
+#pragma unroll
+        for (int b = 0; b < F; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].w, fb[b].w, acc[a][b], 0, 0, 0);
     }
     if (more) {
-      gstore<A_KC>(As[cur ^ 1], ra, tid);
-      gstore<B_KC>(Bs[cur ^ 1], rb, tid);
+      gstore<G, A_KC>(As[cur ^ 1], ra, tid);
+      gstore<G, B_KC>(Bs[cur ^ 1], rb, tid);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < F; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < F; ++b) {
       const int j = n0 + wn + b * 16 + li;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -168,21 +187,31 @@ __global__ __launch_bounds__(kThreads) void k_gemm(Gemm g) {
     }
 }
 
+template <int G>
+void gemm_launch(const Gemm& g, bool a_kc, bool b_kc, int batch, hipStream_t s) {
+  dim3 grid((unsigned)((g.N + G - 1) / G), (unsigned)((g.M + G - 1) / G), (unsigned)batch);
+  if (a_kc && b_kc)
+    hipLaunchKernelGGL((k_gemm<G, true, true>), grid, dim3(kThreads), 0, s, g);
+  else if (a_kc)
+    hipLaunchKernelGGL((k_gemm<G, true, false>), grid, dim3(kThreads), 0, s, g);
+  else if (b_kc)
+    hipLaunchKernelGGL((k_gemm<G, false, true>), grid, dim3(kThreads), 0, s, g);
+  else
+    hipLaunchKernelGGL((k_gemm<G, false, false>), grid, dim3(kThreads), 0, s, g);
+}
+
 int launch_gemm(const Gemm& g, bool a_kc, bool b_kc, int batch, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || batch <= 0) return 0;
   if (batch > 65535) {
     set_error("launch_gemm: %d batched products exceed the grid z limit", batch);
     return CHOL_E_ARG;
   }
-  dim3 grid((unsigned)((g.N + GT - 1) / GT), (unsigned)((g.M + GT - 1) / GT), (unsigned)batch);
-  if (a_kc && b_kc)
-    hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(kThreads), 0, s, g);
-  else if (a_kc)
-    hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(kThreads), 0, s, g);
-  else if (b_kc)
-    hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(kThreads), 0, s, g);
+  // 128-tiles once they alone fill the GPU's 256 CUs twice over.
+  const int64_t tiles128 = (int64_t)((g.N + 127) / 128) * ((g.M + 127) / 128) * batch;
+  if (tiles128 >= 512)
+    gemm_launch<128>(g, a_kc, b_kc, batch, s);
   else
-    hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(kThreads), 0, s, g);
+    gemm_launch<64>(g, a_kc, b_kc, batch, s);
   return launch_status("k_gemm");
 }
 
