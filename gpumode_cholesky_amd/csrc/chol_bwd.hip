@@ -41,6 +41,11 @@ struct Gemm {
   float alpha, beta;
   int64_t sA1, sA2, sB1, sB2, sC1, sC2;
   int nb2;
+  // Triangular operands: the k range of an output tile (rows m0.., columns
+  // n0..) is cut to k >= m0 (klo_m: Aop[m][k] = 0 for k < m), k >= n0
+  // (klo_n: Bop[n][k] = 0 for k < n) and k < m0 + G (khi_m: Aop[m][k] = 0
+  // for k > m); lower_tiles: tiles wholly above the diagonal are not formed.
+  int klo_m, klo_n, khi_m, lower_tiles;
 };
 
 // Loads a G x GKC tile of op(X)[row][k] into registers, G * GKC / 256
@@ -109,6 +114,12 @@ __global__ __launch_bounds__(kThreads) void k_gemm(Gemm g) {
   float* C = g.C + z1 * g.sC1 + z2 * g.sC2;
   const int m0 = blockIdx.y * G;
   const int n0 = blockIdx.x * G;
+  if (g.lower_tiles && n0 >= m0 + G) return;
+  // m0 and n0 are multiples of G, so the range starts on a chunk boundary.
+  int k_lo = 0, k_hi = g.K;
+  if (g.klo_m) k_lo = m0 > k_lo ? m0 : k_lo;
+  if (g.klo_n) k_lo = n0 > k_lo ? n0 : k_lo;
+  if (g.khi_m) k_hi = m0 + G < k_hi ? m0 + G : k_hi;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -124,18 +135,20 @@ __global__ __launch_bounds__(kThreads) void k_gemm(Gemm g) {
     for (int b = 0; b < F; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   float ra[PER], rb[PER];
-  gload<G, A_KC>(ra, A, g.lda, m0, g.M, 0, g.K, tid);
-  gload<G, B_KC>(rb, B, g.ldb, n0, g.N, 0, g.K, tid);
-  gstore<G, A_KC>(As[0], ra, tid);
-  gstore<G, B_KC>(Bs[0], rb, tid);
-  __syncthreads();
-  const int nch = (g.K + GKC - 1) / GKC;
+  const int nch = k_hi > k_lo ? (k_hi - k_lo + GKC - 1) / GKC : 0;
+  if (nch > 0) {
+    gload<G, A_KC>(ra, A, g.lda, m0, g.M, k_lo, k_hi, tid);
+    gload<G, B_KC>(rb, B, g.ldb, n0, g.N, k_lo, k_hi, tid);
+    gstore<G, A_KC>(As[0], ra, tid);
+    gstore<G, B_KC>(Bs[0], rb, tid);
+    __syncthreads();
+  }
   for (int ch = 0; ch < nch; ++ch) {
     const int cur = ch & 1;
     const bool more = ch + 1 < nch;
     if (more) {
-      gload<G, A_KC>(ra, A, g.lda, m0, g.M, (ch + 1) * GKC, g.K, tid);
-      gload<G, B_KC>(rb, B, g.ldb, n0, g.N, (ch + 1) * GKC, g.K, tid);
+      gload<G, A_KC>(ra, A, g.lda, m0, g.M, k_lo + (ch + 1) * GKC, k_hi, tid);
+      gload<G, B_KC>(rb, B, g.ldb, n0, g.N, k_lo + (ch + 1) * GKC, k_hi, tid);
     }
 #pragma unroll
     for (int ks = 0; ks < GKC; ks += 16) {
@@ -336,6 +349,7 @@ int tri_inverse(const float* L, float* X, int64_t batch, int n, float* ws, hipSt
           g.C = ws + (int64_t)c0 * n + a0; g.ldc = n;
           g.M = hc; g.N = h; g.K = h;
           g.alpha = 1.f; g.beta = 0.f;
+          g.klo_n = 1;  // X[A, A] lower triangular
         } else {
           // X[C, A] = -X[C, C] * ws[C, A]
           g.A = X + (int64_t)c0 * n + c0;  g.lda = n;  // Aop[m][k] = X[c0+m][c0+k]
@@ -343,6 +357,7 @@ int tri_inverse(const float* L, float* X, int64_t batch, int n, float* ws, hipSt
           g.C = X + (int64_t)c0 * n + a0;  g.ldc = n;
           g.M = hc; g.N = h; g.K = hc;
           g.alpha = -1.f; g.beta = 0.f;
+          g.khi_m = 1;  // X[C, C] lower triangular
         }
         g.sA1 = g.sB1 = g.sC1 = nn;
         g.sA2 = g.sB2 = g.sC2 = (int64_t)2 * h * n + 2 * h;  // next pair along the diagonal
@@ -376,15 +391,21 @@ int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int 
   g.sA1 = g.sB1 = g.sC1 = nn;
   g.A = L; g.lda = n; g.B = Lbar; g.ldb = n; g.C = P; g.ldc = n;
   g.M = n; g.N = n; g.K = n; g.alpha = 1.f; g.beta = 0.f;
+  g.klo_m = 1;        // L^T upper triangular: k >= i
+  g.lower_tiles = 1;  // Phi reads the lower triangle of P only
   if ((rc = launch_gemm(g, false, false, (int)batch, s))) return rc;
   // 2. S = 1/2 (Phi(P) + Phi(P)^T), in place
   hipLaunchKernelGGL(k_sym_phi, dim3(grid1d(batch * nn)), dim3(256), 0, s, P, n, batch * nn);
   if ((rc = launch_status("k_sym_phi"))) return rc;
   // 3. T = L^-T S:  Aop[i][k] = Linv[k][i], Bop[j][k] = S[j][k]
   g.A = Linv; g.B = P; g.C = T;
+  g.klo_m = 1;  // L^-T upper triangular: k >= i
+  g.lower_tiles = 0;
   if ((rc = launch_gemm(g, false, true, (int)batch, s))) return rc;
   // 4. Abar = T L^-1:  Aop[i][k] = T[i][k], Bop[j][k] = Linv[k][j]
   g.A = T; g.B = Linv; g.C = Abar;
+  g.klo_m = 0;
+  g.klo_n = 1;  // L^-1 lower triangular: Bop[j][k] = Linv[k][j] = 0 for k < j
   if ((rc = launch_gemm(g, true, false, (int)batch, s))) return rc;
   hipLaunchKernelGGL(k_symmetrize, dim3(grid1d(batch * nn)), dim3(256), 0, s, Abar, n, batch * nn);
   return launch_status("k_symmetrize");
