@@ -231,17 +231,24 @@ int launch_gemm(const Gemm& g, bool a_kc, bool b_kc, int batch, hipStream_t s) {
 // Inverse of the 64x64 diagonal blocks of L (rows/cols past n: identity),
 // one wavefront per (matrix, block); lane c forward-substitutes column c.
 // Writes the blocks into X (zeros above the diagonal inside each block).
+// Lane c keeps its column of the inverse in LDS (Ws[i][c], an address of its
+// own, so no barrier is needed between its writes and reads): a register
+// array indexed by the unrolled substitution kept too many values live and
+// spilled.
 __global__ __launch_bounds__(64) void k_leaf_inverse(const float* __restrict__ L, float* __restrict__ X, int n,
                                                      int nblk) {
   constexpr int NB = 64, P = NB + 4;
   __shared__ __align__(16) float Ls[NB * P];
+  __shared__ float Ws[NB * NB];
   const int c = threadIdx.x;
   const int64_t b = blockIdx.x / nblk;
   const int k0 = (blockIdx.x % nblk) * NB;
   const int nb = min(NB, n - k0);
   const float* Lb = L + b * (int64_t)n * n;
   float* Xb = X + b * (int64_t)n * n;
-  // stage the block (row r = lane), identity past nb
+  // stage the block (lane = column), identity past nb; 16 rows' loads in
+  // flight at a time
+#pragma unroll 16
   for (int r = 0; r < NB; ++r) {
     float v;
     if (r < nb)
@@ -251,26 +258,22 @@ __global__ __launch_bounds__(64) void k_leaf_inverse(const float* __restrict__ L
     Ls[r * P + c] = v;
   }
   __syncthreads();
-  float w[NB];
-#pragma unroll
   for (int i = 0; i < NB; ++i) {
-    float s = (i == c) ? 1.f : 0.f;
-    const float* Li = &Ls[i * P];
-#pragma unroll
-    for (int q = 0; q < i; q += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(Li + q);
-      s = fmaf(-v.x, w[q], s);
-      if (q + 1 < i) s = fmaf(-v.y, w[q + 1], s);
-      if (q + 2 < i) s = fmaf(-v.z, w[q + 2], s);
-      if (q + 3 < i) s = fmaf(-v.w, w[q + 3], s);
+    const float* Li = &Ls[i * P];  // row i, read by every lane (broadcast)
+    float s0 = (i == c) ? 1.f : 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int k = 0;
+    for (; k + 4 <= i; k += 4) {
+      const float4 l = *reinterpret_cast<const float4*>(Li + k);
+      s0 = fmaf(-l.x, Ws[k * NB + c], s0);
+      s1 = fmaf(-l.y, Ws[(k + 1) * NB + c], s1);
+      s2 = fmaf(-l.z, Ws[(k + 2) * NB + c], s2);
+      s3 = fmaf(-l.w, Ws[(k + 3) * NB + c], s3);
     }
-    w[i] = s * __builtin_amdgcn_rcpf(Li[i]);
+    for (; k < i; ++k) s0 = fmaf(-Li[k], Ws[k * NB + c], s0);
+    Ws[i * NB + c] = ((s0 + s1) + (s2 + s3)) * __builtin_amdgcn_rcpf(Li[i]);
   }
-  if (c < nb) {
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-      if (i < nb) Xb[(int64_t)(k0 + i) * n + k0 + c] = w[i];
-  }
+  if (c < nb)
+    for (int i = 0; i < nb; ++i) Xb[(int64_t)(k0 + i) * n + k0 + c] = Ws[i * NB + c];
 }
 
 __global__ void k_fill_zero(float* __restrict__ p, int64_t count) {
@@ -372,6 +375,10 @@ int tri_inverse(const float* L, float* X, int64_t batch, int n, float* ws, hipSt
 // (+1 more when Linv is recomputed).
 int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int n, const float* Linv_in,
              float* ws, hipStream_t s) {
+  if (batch > 65535) {  // the batch is a grid dimension of every launch below
+    set_error("chol_backward_f32: batch %lld exceeds 65535", (long long)batch);
+    return CHOL_E_ARG;
+  }
   const int64_t nn = (int64_t)n * n;
   float* P = ws;                 // batch x n x n
   float* T = ws + batch * nn;    // batch x n x n
