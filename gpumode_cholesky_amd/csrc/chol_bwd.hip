@@ -46,23 +46,36 @@ struct Gemm {
   // (klo_n: Bop[n][k] = 0 for k < n) and k < m0 + G (khi_m: Aop[m][k] = 0
   // for k > m); lower_tiles: tiles wholly above the diagonal are not formed.
   int klo_m, klo_n, khi_m, lower_tiles;
+  // Set by launch_gemm: every operand address of the launch is 16-byte
+  // aligned and a multiple of 4 floats apart (float4 global loads).
+  int vec;
 };
 
 // Loads a G x GKC tile of op(X)[row][k] into registers, G * GKC / 256
 // floats per thread.
 template <int G, bool KC_CONTIG>
 __device__ __forceinline__ void gload(float (&r)[G * GKC / kThreads], const float* base, int64_t ld, int row0,
-                                      int rlim, int k0, int klim, int tid) {
+                                      int rlim, int k0, int klim, int tid, bool vec) {
   constexpr int PER = G * GKC / kThreads;  // 8 (G = 64) or 16 (G = 128)
   if constexpr (KC_CONTIG) {
     // thread: row = tid / (GKC / PER), k = (tid % (GKC / PER)) * PER .. +PER-1
     constexpr int TPR = GKC / PER;
     const int gr = row0 + tid / TPR;
     const int kk = (tid % TPR) * PER;
+    const float* src = base + (int64_t)gr * ld + k0 + kk;
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
+    for (int u = 0; u < PER; u += 4) {
       const int gk = k0 + kk + u;
-      r[u] = (gr < rlim && gk < klim) ? base[(int64_t)gr * ld + gk] : 0.f;
+      if (vec && gr < rlim && gk + 3 < klim) {
+        const float4 v = *reinterpret_cast<const float4*>(src + u);
+        r[u] = v.x;
+        r[u + 1] = v.y;
+        r[u + 2] = v.z;
+        r[u + 3] = v.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) r[u + e] = (gr < rlim && gk + e < klim) ? src[u + e] : 0.f;
+      }
     }
   } else {
     // thread: k = tid / (G / PER), rows (tid % (G / PER)) * PER .. +PER-1
@@ -71,10 +84,20 @@ __device__ __forceinline__ void gload(float (&r)[G * GKC / kThreads], const floa
     const int kk = tid / TPK;
     const int rr = (tid % TPK) * PER;
     const int gk = k0 + kk;
+    const float* src = base + (int64_t)gk * ld + row0 + rr;
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
+    for (int u = 0; u < PER; u += 4) {
       const int gr = row0 + rr + u;
-      r[u] = (gr < rlim && gk < klim) ? base[(int64_t)gk * ld + gr] : 0.f;
+      if (vec && gk < klim && gr + 3 < rlim) {
+        const float4 v = *reinterpret_cast<const float4*>(src + u);
+        r[u] = v.x;
+        r[u + 1] = v.y;
+        r[u + 2] = v.z;
+        r[u + 3] = v.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) r[u + e] = (gk < klim && gr + e < rlim) ? src[u + e] : 0.f;
+      }
     }
   }
 }
@@ -137,8 +160,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm(Gemm g) {
   float ra[PER], rb[PER];
   const int nch = k_hi > k_lo ? (k_hi - k_lo + GKC - 1) / GKC : 0;
   if (nch > 0) {
-    gload<G, A_KC>(ra, A, g.lda, m0, g.M, k_lo, k_hi, tid);
-    gload<G, B_KC>(rb, B, g.ldb, n0, g.N, k_lo, k_hi, tid);
+    gload<G, A_KC>(ra, A, g.lda, m0, g.M, k_lo, k_hi, tid, g.vec != 0);
+    gload<G, B_KC>(rb, B, g.ldb, n0, g.N, k_lo, k_hi, tid, g.vec != 0);
     gstore<G, A_KC>(As[0], ra, tid);
     gstore<G, B_KC>(Bs[0], rb, tid);
     __syncthreads();
@@ -147,8 +170,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm(Gemm g) {
     const int cur = ch & 1;
     const bool more = ch + 1 < nch;
     if (more) {
-      gload<G, A_KC>(ra, A, g.lda, m0, g.M, k_lo + (ch + 1) * GKC, k_hi, tid);
-      gload<G, B_KC>(rb, B, g.ldb, n0, g.N, k_lo + (ch + 1) * GKC, k_hi, tid);
+      gload<G, A_KC>(ra, A, g.lda, m0, g.M, k_lo + (ch + 1) * GKC, k_hi, tid, g.vec != 0);
+      gload<G, B_KC>(rb, B, g.ldb, n0, g.N, k_lo + (ch + 1) * GKC, k_hi, tid, g.vec != 0);
     }
 #pragma unroll
     for (int ks = 0; ks < GKC; ks += 16) {
@@ -213,8 +236,12 @@ void gemm_launch(const Gemm& g, bool a_kc, bool b_kc, int batch, hipStream_t s) 
     hipLaunchKernelGGL((k_gemm<G, false, false>), grid, dim3(kThreads), 0, s, g);
 }
 
-int launch_gemm(const Gemm& g, bool a_kc, bool b_kc, int batch, hipStream_t s) {
-  if (g.M <= 0 || g.N <= 0 || batch <= 0) return 0;
+int launch_gemm(const Gemm& g0, bool a_kc, bool b_kc, int batch, hipStream_t s) {
+  if (g0.M <= 0 || g0.N <= 0 || batch <= 0) return 0;
+  Gemm g = g0;
+  const uintptr_t addr = reinterpret_cast<uintptr_t>(g.A) | reinterpret_cast<uintptr_t>(g.B);
+  const int64_t strides = g.lda | g.ldb | g.sA1 | g.sA2 | g.sB1 | g.sB2;
+  g.vec = (addr & 15) == 0 && (strides & 3) == 0;
   if (batch > 65535) {
     set_error("launch_gemm: %d batched products exceed the grid z limit", batch);
     return CHOL_E_ARG;
