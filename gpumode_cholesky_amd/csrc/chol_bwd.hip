@@ -310,32 +310,55 @@ __global__ void k_fill_zero(float* __restrict__ p, int64_t count) {
 
 // S = 1/2 (Phi(P) + Phi(P)^T) from the lower triangle of P, in place
 // (P becomes the full symmetric S).
-__global__ void k_sym_phi(float* __restrict__ P, int n, int64_t total) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = e / ((int64_t)n * n);
-    const int64_t r = e - b * (int64_t)n * n;
-    const int i = (int)(r / n), j = (int)(r % n);
-    if (j > i) continue;
-    float* M = P + b * (int64_t)n * n;
-    const float v = 0.5f * M[(int64_t)i * n + j];
-    M[(int64_t)i * n + j] = v;
-    if (j < i) M[(int64_t)j * n + i] = v;
+// In place, over 32x32 tile pairs (I, J), I >= J, of matrix blockIdx.y,
+// each pair owned by one workgroup and staged through LDS so that the
+// transposed half is read and written a row at a time:
+//   PHI : S = 1/2 (Phi(P) + Phi(P)^T) from P's lower triangle;
+//   !PHI: Abar = 1/2 (Abar + Abar^T) (removes the rounding asymmetry of the
+//         two products).
+template <bool PHI>
+__global__ __launch_bounds__(256) void k_sym_tiles(float* __restrict__ X, int n) {
+  __shared__ float Ta[32][33];
+  __shared__ float Tb[PHI ? 1 : 32][33];
+  const int nt = (n + 31) / 32;
+  const int I = blockIdx.x / nt, J = blockIdx.x - (blockIdx.x / nt) * nt;
+  if (J > I) return;
+  float* M = X + (int64_t)blockIdx.y * n * n;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int a = ty + 8 * r;
+    const int i = I * 32 + a, j = J * 32 + tx;
+    Ta[a][tx] = (i < n && j < n) ? M[(int64_t)i * n + j] : 0.f;
+    if constexpr (!PHI) {
+      const int i2 = J * 32 + a, j2 = I * 32 + tx;
+      Tb[a][tx] = (i2 < n && j2 < n) ? M[(int64_t)i2 * n + j2] : 0.f;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int a = ty + 8 * r, c = tx;
+    // tile (I, J): element (I*32 + a, J*32 + c)
+    const int i = I * 32 + a, j = J * 32 + c;
+    float v;
+    if constexpr (PHI)
+      v = 0.5f * (i >= j ? Ta[a][c] : Ta[c][a]);  // i < j only inside a diagonal tile
+    else
+      v = 0.5f * (Ta[a][c] + Tb[c][a]);
+    if (i < n && j < n) M[(int64_t)i * n + j] = v;
+    if (I != J) {
+      // tile (J, I): element (J*32 + a, I*32 + c), the mirror of (I*32 + c, J*32 + a)
+      const int i2 = J * 32 + a, j2 = I * 32 + c;
+      const float w = PHI ? 0.5f * Ta[c][a] : 0.5f * (Tb[a][c] + Ta[c][a]);
+      if (i2 < n && j2 < n) M[(int64_t)i2 * n + j2] = w;
+    }
   }
 }
 
-// Abar = 1/2 (Abar + Abar^T): removes the rounding asymmetry of the two
-// products (each pair is read and written by one thread).
-__global__ void k_symmetrize(float* __restrict__ X, int n, int64_t total) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = e / ((int64_t)n * n);
-    const int64_t r = e - b * (int64_t)n * n;
-    const int i = (int)(r / n), j = (int)(r % n);
-    if (j >= i) continue;
-    float* M = X + b * (int64_t)n * n;
-    const float v = 0.5f * (M[(int64_t)i * n + j] + M[(int64_t)j * n + i]);
-    M[(int64_t)i * n + j] = v;
-    M[(int64_t)j * n + i] = v;
-  }
+dim3 tile_pairs_grid(int64_t batch, int n) {
+  const int nt = (n + 31) / 32;
+  return dim3((unsigned)(nt * nt), (unsigned)batch);
 }
 
 unsigned grid1d(int64_t total) {
@@ -429,8 +452,8 @@ int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int 
   g.lower_tiles = 1;  // Phi reads the lower triangle of P only
   if ((rc = launch_gemm(g, false, false, (int)batch, s))) return rc;
   // 2. S = 1/2 (Phi(P) + Phi(P)^T), in place
-  hipLaunchKernelGGL(k_sym_phi, dim3(grid1d(batch * nn)), dim3(256), 0, s, P, n, batch * nn);
-  if ((rc = launch_status("k_sym_phi"))) return rc;
+  hipLaunchKernelGGL((k_sym_tiles<true>), tile_pairs_grid(batch, n), dim3(256), 0, s, P, n);
+  if ((rc = launch_status("k_sym_tiles"))) return rc;
   // 3. T = L^-T S:  Aop[i][k] = Linv[k][i], Bop[j][k] = S[j][k]
   g.A = Linv; g.B = P; g.C = T;
   g.klo_m = 1;  // L^-T upper triangular: k >= i
@@ -441,8 +464,8 @@ int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int 
   g.klo_m = 0;
   g.klo_n = 1;  // L^-1 lower triangular: Bop[j][k] = Linv[k][j] = 0 for k < j
   if ((rc = launch_gemm(g, true, false, (int)batch, s))) return rc;
-  hipLaunchKernelGGL(k_symmetrize, dim3(grid1d(batch * nn)), dim3(256), 0, s, Abar, n, batch * nn);
-  return launch_status("k_symmetrize");
+  hipLaunchKernelGGL((k_sym_tiles<false>), tile_pairs_grid(batch, n), dim3(256), 0, s, Abar, n);
+  return launch_status("k_sym_tiles");
 }
 
 }  // namespace chol
