@@ -46,8 +46,11 @@ struct TuneRow {
 static std::mutex g_tune_mu;
 static std::vector<TuneRow> g_tune;
 
-// Default when no row matches: LDS kernel up to 256, then outer blocks of
-// 256 columns (512 from n = 12288, where the trailing matrix outgrows the
+// Default when no row matches: the blocked path with one outer block up to
+// n = 256 (measured 1.2-2.2x faster than the whole-matrix LDS kernel from
+// 16x100 to 4096x128 and 512x256), the LDS kernel only for batches of 8192
+// and more (8192x96: 663 us against 721 us), then outer blocks of 256
+// columns (512 from n = 12288, where the trailing matrix outgrows the
 // 256 MiB Infinity Cache and the SYRK's C traffic dominates).
 static int64_t pick_nbo(int64_t n, int64_t batch) {
   {
@@ -55,7 +58,7 @@ static int64_t pick_nbo(int64_t n, int64_t batch) {
     for (const TuneRow& r : g_tune)
       if (n >= r.n_lo && n <= r.n_hi && batch >= r.b_lo && batch <= r.b_hi) return r.nbo;
   }
-  if (n <= kLdsMaxN) return -1;
+  if (n <= kLdsMaxN) return batch >= 8192 ? -1 : (n <= kNB ? kNB : 2 * kNB);
   return n >= 12288 ? 512 : 256;
 }
 

@@ -82,17 +82,20 @@ def test_blocked_families(oracle_mod, n, case, cond):
     _check_against_oracle(oracle_mod, a, f"blocked n={n} {case}")
 
 
+@pytest.mark.parametrize("nbo", [128, -1])
 @pytest.mark.parametrize("n", [65, 100, 127, 128, 129, 200, 256])
-def test_blocked_route_forced(oracle_mod, n):
-    # The blocked route where the tuner could pick it for small orders: one
-    # 128-block (n <= 128) is staged from A by the diagonal-block kernel with
-    # no copy before it; n > 128 ends with a block that forms no inverse.
+def test_small_order_routes_forced(oracle_mod, n, nbo):
+    # Both routes for 64 < n <= 256, forced through the table.  Blocked (the
+    # default below a batch of 8192): one 128-block (n <= 128) is staged from
+    # A by the diagonal-block kernel with no copy before it; n > 128 ends
+    # with a block that forms no inverse.  -1: the whole-matrix LDS kernel
+    # (the default from a batch of 8192).
     from gpumode_cholesky_amd import autotune
     try:
-        autotune.set_table([[n, n, 1, 1 << 20, 128]])
+        autotune.set_table([[n, n, 1, 1 << 20, nbo]])
         for case, cond in (("dense", 2), ("spectrum", 5)):
             a = harness.generate_input(3, n, cond, 5000 + n, case, device="cpu")
-            _check_against_oracle(oracle_mod, a, f"blocked-forced n={n} {case}")
+            _check_against_oracle(oracle_mod, a, f"route nbo={nbo} n={n} {case}")
     finally:
         autotune.load_table()
 
@@ -103,11 +106,13 @@ def test_output_written_whole(n, mode):
     # The kernels write every element of L, zeros above the diagonal
     # included (there is no separate zero fill), so an output buffer full of
     # NaN comes back as the same factor as a fresh one.  n <= 256 also runs
-    # with the blocked route forced.
+    # with the blocked route and (64 < n) the LDS kernel forced.
     from gpumode_cholesky_amd import autotune
     a = harness.generate_input(3, n, 3, 7100 + n, "dense", device="cpu").to(DEV)
     ref = cholesky(a, mode=mode)
     routes = [None] if n > 256 else [None, [[n, n, 1, 1 << 20, 128]]]
+    if 64 < n <= 256:
+        routes.append([[n, n, 1, 1 << 20, -1]])
     try:
         for table in routes:
             if table is not None:
