@@ -6,8 +6,9 @@
 // (batch, n) takes a HIP route and there is no library fallback):
 //   n <= 64        : chol_small.hip, whole matrices in wavefront registers /
 //                    LDS;
-//   64 < n <= 256  : chol_lds.hip, whole matrix in one workgroup's LDS, or
-//                    the blocked path where the tuning table says so;
+//   64 < n <= 256  : the blocked path below a batch of 8192, else
+//                    chol_lds.hip, whole matrix in one workgroup's LDS (the
+//                    tuning table can pick either);
 //   n >  256       : the blocked path: 128-wide diagonal blocks with explicit
 //                    inverses (chol_diag.hip), MFMA panels and SYRK
 //                    (chol_gemm.hip, chol_syrk256.hip), driven by
