@@ -135,8 +135,11 @@ __global__ __launch_bounds__(kThreads) void k_gemm(Gemm g) {
   const float* A = g.A + z1 * g.sA1 + z2 * g.sA2;
   const float* B = g.B + z1 * g.sB1 + z2 * g.sB2;
   float* C = g.C + z1 * g.sC1 + z2 * g.sC2;
-  const int m0 = blockIdx.y * G;
-  const int n0 = blockIdx.x * G;
+  // The slow grid dimension walks the tiles whose K range shrinks along it
+  // (rows for klo_m, columns for klo_n), so the longest tiles of a matrix
+  // are dispatched first and the short ones fill the tail.
+  const int m0 = (g.klo_n ? blockIdx.x : blockIdx.y) * G;
+  const int n0 = (g.klo_n ? blockIdx.y : blockIdx.x) * G;
   if (g.lower_tiles && n0 >= m0 + G) return;
   // m0 and n0 are multiples of G, so the range starts on a chunk boundary.
   int k_lo = 0, k_hi = g.K;
@@ -225,7 +228,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm(Gemm g) {
 
 template <int G>
 void gemm_launch(const Gemm& g, bool a_kc, bool b_kc, int batch, hipStream_t s) {
-  dim3 grid((unsigned)((g.N + G - 1) / G), (unsigned)((g.M + G - 1) / G), (unsigned)batch);
+  const unsigned tn = (unsigned)((g.N + G - 1) / G), tm = (unsigned)((g.M + G - 1) / G);
+  dim3 grid(g.klo_n ? tm : tn, g.klo_n ? tn : tm, (unsigned)batch);
   if (a_kc && b_kc)
     hipLaunchKernelGGL((k_gemm<G, true, true>), grid, dim3(kThreads), 0, s, g);
   else if (a_kc)
