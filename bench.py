@@ -14,9 +14,13 @@ geometric mean of the 15 per-shape means (microseconds, lower is better).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|mixed|both]
 
-Both forms are timed by default; the headline value is the mixed form
-(BF16x3 trailing updates), reported with its FP64 residual beside the plain
-all-FP32 form's.
+Both forms are timed by default.  The headline value is the mixed form
+(``cholesky(A, mode="mixed")``: BF16x3 panel and trailing products), the
+form the metric's mixed-vs-FP32 speedup is about; the drop-in
+``custom_kernel`` runs the plain all-FP32 form, whose geomean is reported
+beside it (``geomean_us.plain``), with each form's FP64 residual.
+
+    python bench.py --paired     # A-B-B-A plain vs mixed per shape (one JSON line)
 """
 from __future__ import annotations
 
@@ -46,7 +50,8 @@ def _parse():
     ap.add_argument("--mode", default="both", choices=["plain", "mixed", "both"])
     ap.add_argument("--shapes", default="", help="comma list like 4096x32,1x4096 (default: all 15)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--paired", action="store_true",
+                    help="same-process A-B-B-A timing of plain (A) vs mixed (B) per shape instead")
     ap.add_argument("--detail", default="", help="write per-shape JSON here")
     ap.add_argument("--no-extras", action="store_true", help="skip ragged / graph / backward measurements")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -62,53 +67,51 @@ def _specs(filter_s: str):
     return specs
 
 
-def cpu_baseline(budget_s: float):
-    """Time the FP64 C oracle (the CPU port of the path) on the host cores.
+def cpu_baseline(specs, data, host_L):
+    """torch.linalg.cholesky in FP32 on the host cores: the reference's own
+    path on a machine without a GPU (custom_kernel falls through to
+    torch.linalg.cholesky_ex there, submission.py:4109, :4268).
 
-    Sample: per shape, as many matrices of that shape as fit a share of the
-    budget (at least one) for n <= 2048; n >= 4096 is extrapolated from one
-    n = 2048 matrix by (n/2048)^3.  Per-shape time = per-matrix time x batch;
-    the value is the geometric mean over the 15 shapes, as for the GPU.
+    Same inputs as the GPU run (input 0 of each shape, copied to the host),
+    every shape measured: median of 5 runs after a warmup for n <= 2048, of 3
+    for n <= 8192, one run above.  Also gives, per shape and form, the
+    max-norm forward error of the GPU factor against this one.
     """
-    import numpy as np
-
-    import oracle
-
-    oracle.build()
     threads = os.cpu_count() or 1
-    per_matrix = {}
-    sample = []
-    share = budget_s / 10.0
-    for spec in harness.BENCH_SPECS:
+    torch.set_num_threads(threads)
+    shape_us, fwd, sample = [], {}, []
+    for spec in specs:
         b, n = spec["batch"], spec["n"]
-        if n > 2048 or n in per_matrix:
-            continue
-        a = harness.config_input(1, n, spec["seed"], device="cpu").numpy()
-        t0 = time.perf_counter()
-        oracle.potrf64(a)
-        one = time.perf_counter() - t0
-        k = int(max(1, min(b, share / max(one, 1e-6))))
-        batch_a = np.repeat(a, k, axis=0)
-        t0 = time.perf_counter()
-        oracle.potrf64(batch_a)
-        per_matrix[n] = (time.perf_counter() - t0) / k
-        sample.append(f"{k}x{n}")
-    shape_us = []
-    for spec in harness.BENCH_SPECS:
-        b, n = spec["batch"], spec["n"]
-        t = per_matrix[n] if n in per_matrix else per_matrix[2048] * (n / 2048.0) ** 3
-        shape_us.append(t * b * 1e6)
+        a = data[(b, n)][0].cpu()
+        reps = 5 if n <= 2048 else (3 if n <= 8192 else 1)
+        if n <= 2048:
+            torch.linalg.cholesky(a)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            Lr = torch.linalg.cholesky(a)
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        us = ts[len(ts) // 2] * 1e6
+        shape_us.append(us)
+        sample.append(f"{b}x{n}:{reps}")
+        scale = Lr.abs().max().item()
+        fwd[(b, n)] = {m: (L - Lr).abs().max().item() / scale for m, L in host_L.get((b, n), {}).items()}
+        print(f"# cpu {b}x{n}: {us:.0f} us (median of {reps}) fwd_vs_torch_cpu "
+              + " ".join(f"{m}={e:.2e}" for m, e in fwd[(b, n)].items()), file=sys.stderr, flush=True)
+        del a, Lr
     geo = math.exp(sum(math.log(x) for x in shape_us) / len(shape_us))
-    return {
+    out = {
         "value": geo,
-        "unit": "us (geomean over 15 shapes)",
+        "unit": "us (geomean over the shapes)",
         "cores": threads,
-        "kind": "port",
-        "sample": ("FP64 C oracle (oracle/cholesky_oracle.c, OpenMP over the batch) on "
-                   + ", ".join(sample) + " matrices; per-shape time = per-matrix time x batch; "
-                   "n >= 4096 extrapolated from n = 2048 by n^3"),
+        "kind": "reference",
+        "sample": ("torch.linalg.cholesky FP32 (LAPACK spotrf via torch, the reference's custom_kernel on a "
+                   "machine without a GPU) on input 0 of every shape, all measured: " + ", ".join(sample)
+                   + " runs (median)"),
         "per_shape_us": [round(x, 1) for x in shape_us],
     }
+    return out, fwd
 
 
 def _inputs(specs, dev):
@@ -121,9 +124,10 @@ def _inputs(specs, dev):
     return data
 
 
-def _gate(specs, data, mode):
+def _gate(specs, data, mode, host_L=None):
     """Correctness of one form on every shape (untimed): the reference's gate
-    and the FP64 residual ||LL^T - A||_F / ||A||_F."""
+    and the FP64 residual ||LL^T - A||_F / ||A||_F.  host_L (a dict) receives
+    a host copy of each factor for the forward-error comparison."""
     out = {}
     for spec in specs:
         key = (spec["batch"], spec["n"])
@@ -135,6 +139,8 @@ def _gate(specs, data, mode):
             print(json.dumps({"error": f"{mode} {key[0]}x{key[1]}: {msg}"}))
             sys.exit(1)
         out[key] = harness.fp64_residual(a, L)
+        if host_L is not None:
+            host_L.setdefault(key, {})[mode] = L.cpu()
         del L
         print(f"# gate {mode} {key[0]}x{key[1]}: {msg} fp64_resid={out[key]:.2e}",
               file=sys.stderr, flush=True)
@@ -221,7 +227,7 @@ def extras(dev):
         ins = [harness.config_input(b, n, 31 + i, device=dev) for i in range(8)]
         g = GraphedCholesky(ins)
         eager = _event_us(lambda: [cholesky(a) for a in ins], 5) / len(ins)
-        replay = _event_us(lambda: [g.replay(i) for i in range(len(g))], 5) / len(ins)
+        replay = _event_us(g.replay, 5) / len(ins)
         gr[f"{b}x{n}"] = {"eager_us": round(eager, 2), "graph_us": round(replay, 2)}
         del g, ins
     out["graph"] = gr
@@ -254,6 +260,28 @@ def extras(dev):
     return out
 
 
+def paired(specs, dev):
+    """A-B-B-A same-process comparison, plain (A) against mixed (B), per
+    shape (scripts/_gpu_runner.py:2494-2588): median ratio plain/mixed with
+    its bootstrap 95 % interval; one JSON line."""
+    rows = []
+    for spec in specs:
+        b, n = spec["batch"], spec["n"]
+        k, iters, repeats = harness.paired_budget(n)
+        data = [harness.config_input(b, n, spec["seed"] + 42 * i, device=dev) for i in range(k)]
+        r = harness.paired_compare(lambda d: cholesky(d, mode="plain"), lambda d: cholesky(d, mode="mixed"),
+                                   data, iters, repeats, warmup=3 if n <= 2048 else 2)
+        r.update({"batch": b, "n": n})
+        rows.append(r)
+        print(f"# paired {b}x{n}: plain/mixed {r['ratio_median']:.3f} "
+              f"[{r['ratio_ci95'][0]:.3f}, {r['ratio_ci95'][1]:.3f}] plain {r['a_us']:.1f} us mixed {r['b_us']:.1f} us",
+              file=sys.stderr, flush=True)
+        del data
+        torch.cuda.empty_cache()
+    print(json.dumps({"mode": "paired", "a": "plain", "b": "mixed",
+                      "geomean_ratio": _geo(r["ratio_median"] for r in rows), "rows": rows}))
+
+
 def _geo(xs):
     xs = list(xs)
     return math.exp(sum(math.log(x) for x in xs) / len(xs))
@@ -275,9 +303,13 @@ def main():
         dst.copy_(src)  # reads and writes exactly copy_probe MiB
         torch.cuda.synchronize()
         del src, dst
+    if args.paired:
+        paired(specs, dev)
+        return
     modes = ["plain", "mixed"] if args.mode == "both" else [args.mode]
     data = _inputs(specs, dev)
-    resid = {m: _gate(specs, data, m) for m in modes}
+    host_L = {} if not args.no_cpu_baseline else None
+    resid = {m: _gate(specs, data, m, host_L) for m in modes}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     times = {m: _time(specs, data, m, args.steps, args.warmup, dev) for m in modes}
@@ -317,8 +349,14 @@ def main():
                 "achieved": g, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(g / HBM_PEAK_GBS, 4), "traffic": PMC_TRAFFIC_4096x32,
                 "algorithmic_bytes": 8 * 4096 * 32 * 32}
+    cpu = fwd = None
+    if not args.no_cpu_baseline:
+        cpu, fwd = cpu_baseline(specs, data, host_L)
+        for row in rows:
+            for m in modes:
+                row[m]["fwd_vs_torch_cpu"] = fwd[(row["batch"], row["n"])].get(m)
     out = {
-        "metric": "geomean us over 15 batch x n SPD shapes",
+        "metric": "geomean us over 15 batch x n SPD shapes (mixed form; plain in geomean_us)",
         "value": round(geo[head], 2),
         "unit": "us",
         "n_gpus": 1,
@@ -331,7 +369,8 @@ def main():
         "dtype": "f32" if head == "plain" else "f32+bf16x3",
         "data": "synthetic",
         "config": {"workload": "15-shape batched Cholesky grid (4096x32 ... 1x32768), FP32 in/out",
-                   "form": head, "shapes": len(rows)},
+                   "form": head, "value_form": f"cholesky(A, mode='{head}')",
+                   "drop_in_form": "custom_kernel(A) = cholesky(A, mode='plain')", "shapes": len(rows)},
         "roofline": roof,
         "geomean_us": {m: round(geo[m], 2) for m in modes},
         "max_fp64_resid": {m: max(resid[m].values()) for m in modes},
@@ -342,8 +381,8 @@ def main():
         out["mixed_speedup_vs_plain"] = round(geo["plain"] / geo["mixed"], 4)
     if not args.no_extras:
         out["extras"] = extras(dev)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_budget_s)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     if args.detail:
         with open(args.detail, "w") as f:
             json.dump(out, f, indent=1)

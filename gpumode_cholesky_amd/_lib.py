@@ -37,6 +37,9 @@ SIGNATURES = {
     "chol_backward_f32": (ctypes.c_int, [_p, _p, _p, _i64, _i64, _p, _i32, _p, _sz, _p]),
     "chol_tri_inverse_workspace_bytes": (_sz, [_i64, _i64]),
     "chol_tri_inverse_f32": (ctypes.c_int, [_p, _p, _i64, _i64, _p, _sz, _p]),
+    "chol_route": (ctypes.c_int, [_i64, _i64, _i32, _p]),
+    "chol_plan_potrf": (ctypes.c_int, [_i64, _i64, _i64, _i32, _i32, _p, _i64, _p]),
+    "chol_route_counters": (_i64, [_p, _i64]),
 }
 
 

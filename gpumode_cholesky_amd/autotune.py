@@ -26,8 +26,11 @@ NBO_CANDIDATES = (128, 256, 512, 1024, 2048)
 
 
 def set_table(rows) -> None:
-    """Install rows [n_lo, n_hi, batch_lo, batch_hi, nbo] (first match wins)."""
-    arr = np.ascontiguousarray(np.asarray(rows, dtype=np.int64).reshape(-1, 5))
+    """Install rows [n_lo, n_hi, batch_lo, batch_hi, mode, nbo] (mode 0 plain,
+    1 mixed, -1 both; a 5-element row applies to both forms).  First match
+    wins."""
+    rows = [list(r) if len(r) == 6 else list(r[:4]) + [-1, r[4]] for r in rows]
+    arr = np.ascontiguousarray(np.asarray(rows, dtype=np.int64).reshape(-1, 6))
     _lib.check(_lib.load().chol_set_tuning(arr.ctypes.data if len(arr) else None, len(arr)),
                "chol_set_tuning")
 
@@ -70,9 +73,11 @@ def time_config(a: torch.Tensor, mode: str, reps: int = 5) -> float:
 
 
 def tune(shapes, mode: str = "plain", reps: int = 5, log=print) -> list:
-    """Time every candidate for each (batch, n); return the winning rows."""
+    """Time every candidate for each (batch, n) in one form; return the
+    winning rows (tagged with that form)."""
     from .harness import config_input
 
+    mcode = 1 if mode == "mixed" else 0
     rows = []
     dev = torch.device("cuda", torch.cuda.current_device())
     for batch, n in shapes:
@@ -86,7 +91,7 @@ def tune(shapes, mode: str = "plain", reps: int = 5, log=print) -> list:
             log(f"tune {batch}x{n} {mode} nbo={nbo}: {us:.1f} us")
             if best is None or us < best[1]:
                 best = (nbo, us)
-        rows.append([n, n, batch, batch, best[0]])
+        rows.append([n, n, batch, batch, mcode, best[0]])
         log(f"tune {batch}x{n} {mode}: best nbo={best[0]} ({best[1]:.1f} us)")
         del a
         torch.cuda.empty_cache()
@@ -108,9 +113,15 @@ def main():
     else:
         shapes = [(s["batch"], s["n"]) for s in BENCH_SPECS]
     rows = tune(shapes, args.mode, args.reps, log=lambda m: print(m, flush=True))
+    # keep the other form's rows of an existing table
+    mcode = 1 if args.mode == "mixed" else 0
+    keep = []
+    if os.path.exists(args.out):
+        with open(args.out) as f:
+            keep = [r for r in json.load(f)["rows"] if len(r) == 6 and r[4] != mcode]
     with open(args.out, "w") as f:
-        json.dump({"mode": args.mode, "rows": rows}, f, indent=1)
-    print(f"wrote {len(rows)} rows to {args.out}")
+        json.dump({"columns": ["n_lo", "n_hi", "batch_lo", "batch_hi", "mode", "nbo"], "rows": keep + rows}, f)
+    print(f"wrote {len(rows)} {args.mode} rows to {args.out} ({len(keep)} kept)")
 
 
 if __name__ == "__main__":

@@ -47,20 +47,33 @@ def workspace_bytes(batch: int, n: int, mode: str = "plain") -> int:
     return int(_lib.load().chol_workspace_bytes(batch, n, _MODES[mode]))
 
 
+# Matrices the mixed form handed to the plain form (fallback=True), since load.
+MIXED_FALLBACKS = 0
+
+
 def cholesky(A: torch.Tensor, mode: str = "plain", return_info: bool = False,
-             out: torch.Tensor | None = None):
+             out: torch.Tensor | None = None, fallback: bool = False):
     """Lower Cholesky factor of each matrix of ``A`` ([..., n, n] FP32).
 
     mode: ``"plain"`` (all-FP32) or ``"mixed"`` (BF16x3 split products in the
     panel solves and trailing updates of the blocked path, which runs for
     n > 256 and for 64 < n <= 256 where the tuning table picks it; the
     register and LDS kernels are the same in both forms).
+    fallback (mixed form): BF16x3 products carry ~2^-16 relative error, so on
+    ill-conditioned input (cond ~1e5 and beyond, e.g. a low-rank matrix plus
+    1e-4 I) a pivot can turn non-positive.  With fallback=True the call reads
+    info back (one host synchronisation) and refactors those matrices in the
+    plain form, as the reference's isfinite fallback chain does
+    (submission.py:4158-4164, :4268); without it they come back as NaN with
+    info set, like any non-SPD input.
     Returns ``L`` or ``(L, info)`` with ``info`` int32 [...] on the device.
     """
     m = _MODES.get(mode)
     if m is None:
         raise ValueError(f"mode must be one of {sorted(_MODES)}, got {mode!r}")
     _validate(A)
+    if fallback and m == _lib.MODE_MIXED:
+        return _mixed_with_fallback(A, return_info, out)
     lib = _lib.load()
     shape = A.shape
     n = shape[-1]
@@ -92,6 +105,21 @@ def cholesky(A: torch.Tensor, mode: str = "plain", return_info: bool = False,
     if return_info:
         return L, info
     return L
+
+
+def _mixed_with_fallback(A, return_info, out):
+    global MIXED_FALLBACKS
+    L, info = cholesky(A, mode="mixed", return_info=True, out=out)
+    bad = info.reshape(-1) != 0
+    if bad.any().item():
+        n = A.shape[-1]
+        idx = bad.nonzero().squeeze(1)
+        sub = A.reshape(-1, n, n)[idx]
+        Lp, ip = cholesky(sub, mode="plain", return_info=True)
+        L.view(-1, n, n)[idx] = Lp
+        info.view(-1)[idx] = ip
+        MIXED_FALLBACKS += int(idx.numel())
+    return (L, info) if return_info else L
 
 
 def custom_kernel(data: torch.Tensor) -> torch.Tensor:

@@ -240,23 +240,38 @@ void gemm_launch(const Gemm& g, bool a_kc, bool b_kc, int batch, hipStream_t s) 
     hipLaunchKernelGGL((k_gemm<G, false, false>), grid, dim3(kThreads), 0, s, g);
 }
 
-int launch_gemm(const Gemm& g0, bool a_kc, bool b_kc, int batch, hipStream_t s) {
-  if (g0.M <= 0 || g0.N <= 0 || batch <= 0) return 0;
+// zcount products (matrices x nb2 sub-blocks) on grid.z, which holds at most
+// 65535: launches of whole matrices each, the operand pointers advanced by
+// the matrices before them.
+int launch_gemm(const Gemm& g0, bool a_kc, bool b_kc, int64_t zcount, hipStream_t s) {
+  if (g0.M <= 0 || g0.N <= 0 || zcount <= 0) return 0;
   Gemm g = g0;
   const uintptr_t addr = reinterpret_cast<uintptr_t>(g.A) | reinterpret_cast<uintptr_t>(g.B);
   const int64_t strides = g.lda | g.ldb | g.sA1 | g.sA2 | g.sB1 | g.sB2;
   g.vec = (addr & 15) == 0 && (strides & 3) == 0;
-  if (batch > 65535) {
-    set_error("launch_gemm: %d batched products exceed the grid z limit", batch);
+  const int64_t per = (int64_t)(65535 / g.nb2) * g.nb2;
+  if (g.nb2 <= 0 || per <= 0 || zcount % g.nb2 != 0) {
+    set_error("launch_gemm: %lld products do not split into matrices of %d", (long long)zcount, g.nb2);
     return CHOL_E_ARG;
   }
-  // 128-tiles once they alone fill the GPU's 256 CUs twice over.
-  const int64_t tiles128 = (int64_t)((g.N + 127) / 128) * ((g.M + 127) / 128) * batch;
-  if (tiles128 >= 512)
-    gemm_launch<128>(g, a_kc, b_kc, batch, s);
-  else
-    gemm_launch<64>(g, a_kc, b_kc, batch, s);
-  return launch_status("k_gemm");
+  // 128-tiles once they alone fill the GPU's 256 CUs twice over (decided on
+  // the whole product count, so the tile shape does not depend on the split).
+  const int64_t tiles128 = (int64_t)((g.N + 127) / 128) * ((g.M + 127) / 128) * zcount;
+  for (int64_t z0 = 0; z0 < zcount; z0 += per) {
+    Gemm c = g;
+    const int64_t m0 = z0 / g.nb2;
+    c.A += m0 * g.sA1;
+    c.B += m0 * g.sB1;
+    c.C += m0 * g.sC1;
+    const int cnt = (int)((zcount - z0) < per ? (zcount - z0) : per);
+    if (tiles128 >= 512)
+      gemm_launch<128>(c, a_kc, b_kc, cnt, s);
+    else
+      gemm_launch<64>(c, a_kc, b_kc, cnt, s);
+    const int rc = launch_status("k_gemm");
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 // Inverse of the 64x64 diagonal blocks of L (rows/cols past n: identity),
@@ -360,9 +375,18 @@ __global__ __launch_bounds__(256) void k_sym_tiles(float* __restrict__ X, int n)
   }
 }
 
-dim3 tile_pairs_grid(int64_t batch, int n) {
+// k_sym_tiles over a batch, at most 65535 matrices (grid.y) per launch.
+template <bool PHI>
+int sym_tiles(float* X, int64_t batch, int n, hipStream_t s) {
   const int nt = (n + 31) / 32;
-  return dim3((unsigned)(nt * nt), (unsigned)batch);
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+    const int64_t cnt = (batch - b0) < 65535 ? (batch - b0) : 65535;
+    hipLaunchKernelGGL((k_sym_tiles<PHI>), dim3((unsigned)(nt * nt), (unsigned)cnt), dim3(256), 0, s, X + b0 * nn, n);
+    const int rc = launch_status("k_sym_tiles");
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 unsigned grid1d(int64_t total) {
@@ -418,7 +442,7 @@ int tri_inverse(const float* L, float* X, int64_t batch, int n, float* ws, hipSt
         }
         g.sA1 = g.sB1 = g.sC1 = nn;
         g.sA2 = g.sB2 = g.sC2 = (int64_t)2 * h * n + 2 * h;  // next pair along the diagonal
-        if ((rc = launch_gemm(g, true, false, (int)(batch * cnt), s))) return rc;
+        if ((rc = launch_gemm(g, true, false, batch * cnt, s))) return rc;
       }
     }
   }
@@ -429,10 +453,6 @@ int tri_inverse(const float* L, float* X, int64_t batch, int n, float* ws, hipSt
 // (+1 more when Linv is recomputed).
 int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int n, const float* Linv_in,
              float* ws, hipStream_t s) {
-  if (batch > 65535) {  // the batch is a grid dimension of every launch below
-    set_error("chol_backward_f32: batch %lld exceeds 65535", (long long)batch);
-    return CHOL_E_ARG;
-  }
   const int64_t nn = (int64_t)n * n;
   float* P = ws;                 // batch x n x n
   float* T = ws + batch * nn;    // batch x n x n
@@ -454,22 +474,20 @@ int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int 
   g.M = n; g.N = n; g.K = n; g.alpha = 1.f; g.beta = 0.f;
   g.klo_m = 1;        // L^T upper triangular: k >= i
   g.lower_tiles = 1;  // Phi reads the lower triangle of P only
-  if ((rc = launch_gemm(g, false, false, (int)batch, s))) return rc;
+  if ((rc = launch_gemm(g, false, false, batch, s))) return rc;
   // 2. S = 1/2 (Phi(P) + Phi(P)^T), in place
-  hipLaunchKernelGGL((k_sym_tiles<true>), tile_pairs_grid(batch, n), dim3(256), 0, s, P, n);
-  if ((rc = launch_status("k_sym_tiles"))) return rc;
+  if ((rc = sym_tiles<true>(P, batch, n, s))) return rc;
   // 3. T = L^-T S:  Aop[i][k] = Linv[k][i], Bop[j][k] = S[j][k]
   g.A = Linv; g.B = P; g.C = T;
   g.klo_m = 1;  // L^-T upper triangular: k >= i
   g.lower_tiles = 0;
-  if ((rc = launch_gemm(g, false, true, (int)batch, s))) return rc;
+  if ((rc = launch_gemm(g, false, true, batch, s))) return rc;
   // 4. Abar = T L^-1:  Aop[i][k] = T[i][k], Bop[j][k] = Linv[k][j]
   g.A = T; g.B = Linv; g.C = Abar;
   g.klo_m = 0;
   g.klo_n = 1;  // L^-1 lower triangular: Bop[j][k] = Linv[k][j] = 0 for k < j
-  if ((rc = launch_gemm(g, true, false, (int)batch, s))) return rc;
-  hipLaunchKernelGGL((k_sym_tiles<false>), tile_pairs_grid(batch, n), dim3(256), 0, s, Abar, n);
-  return launch_status("k_sym_tiles");
+  if ((rc = launch_gemm(g, true, false, batch, s))) return rc;
+  return sym_tiles<false>(Abar, batch, n, s);
 }
 
 }  // namespace chol

@@ -15,6 +15,10 @@ namespace chol {
 void set_error(const char* fmt, ...);
 void clear_error();
 
+// Successful calls per route, read by chol_route_counters (in its order).
+enum RouteCounter { RC_SMALL, RC_LDS, RC_BLOCKED, RC_RAGGED, RC_BACKWARD, RC_TRI_INVERSE, RC_COUNT };
+void note_call(RouteCounter r);
+
 // Wave size of CDNA: every kernel here is written for 64-lane wavefronts.
 constexpr int kWave = 64;
 
@@ -294,5 +298,13 @@ int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int 
 
 // Diagonal block width of the blocked path.
 constexpr int kNB = 128;
+
+// Dense dispatch behind chol_potrf_f32 (argument checks, route, launches);
+// side_ok = false keeps all of the blocked route on stream s.  `who` names
+// the entry point in error messages.
+int potrf_dispatch(const char* who, const float* A, float* L, int64_t batch, int64_t n, int64_t batch_stride,
+                   int64_t ld, int32_t mode, int32_t* info, void* ws, size_t ws_bytes, hipStream_t s, bool side_ok);
+// Device workspace bytes of the dense dispatch (chol_workspace_bytes).
+size_t potrf_workspace(int64_t batch, int64_t n);
 
 }  // namespace chol

@@ -140,19 +140,25 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
   };
   const int64_t c64 = first_at_most(64), c32 = first_at_most(32), c16 = first_at_most(16), cz = first_at_most(0);
 
-  // n > 64: lock-step blocked path over the prefix [0, c64).
-  if (c64 > 0) {
+  // n > 64: lock-step blocked path over the prefix [0, c64), in chunks of at
+  // most 65535 matrices (the kernels put the matrix on grid.y).
+  for (int64_t c0 = 0; c0 < c64; c0 += 65535) {
+    const int64_t cnt = std::min<int64_t>(c64 - c0, 65535);
+    const int32_t* cns = ns + c0;  // host copy of the chunk's orders (sorted, largest first)
     Batch q = base;
-    q.batch = c64;
-    q.n = ns[0];
-    float* winv = reinterpret_cast<float*>(w + lay.winv);
+    q.Ap = base.Ap + c0;
+    q.Lp = base.Lp + c0;
+    q.ns = base.ns + c0;
+    q.orig = base.orig + c0;
+    q.batch = cnt;
+    q.n = cns[0];
+    float* winv = reinterpret_cast<float*>(w + lay.winv) + c0 * kNB * kNB;
     if ((rc = launch_copy_lower(q, s))) return rc;
-    for (int k0 = 0; k0 < ns[0]; k0 += kNB) {
+    for (int k0 = 0; k0 < cns[0]; k0 += kNB) {
       int64_t act = 0, act2 = 0;
-      while (act < c64 && ns[act] > k0) ++act;
-      while (act2 < c64 && ns[act2] > k0 + kNB) ++act2;
+      while (act < cnt && cns[act] > k0) ++act;
+      while (act2 < cnt && cns[act2] > k0 + kNB) ++act2;
       q.batch = act;
-      q.n = ns[0];
       if ((rc = launch_diag_factor(q, k0, winv, s))) return rc;
       if (act2 == 0) break;
       q.batch = act2;
@@ -171,6 +177,7 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
   if ((rc = launch_small_queue(base, c16, cz - c16, 16, queues + 4,
                                (int)std::min<int64_t>((waves(cz - c16, 16) + 3) / 4, 1024), s)))
     return rc;
+  note_call(RC_RAGGED);
   return 0;
 }
 

@@ -289,3 +289,153 @@ def benchmark(fn, data_list, max_repeats: int = 200, max_time_s: float = 10.0,
             if st.err / st.mean < 0.001 or st.mean * 1e-6 * st.runs > max_time_s or elapsed > 120:
                 break
     return calculate_stats(durations)
+
+
+def _median(values):
+    s = sorted(values)
+    m = len(s) // 2
+    return s[m] if len(s) % 2 else 0.5 * (s[m - 1] + s[m])
+
+
+def _bootstrap_ci(values, iters: int = 4000, seed: int = 20350719, alpha: float = 0.05):
+    """Percentile bootstrap interval of the median (scripts/_gpu_runner.py:2438-2452)."""
+    import random
+    if len(values) < 2:
+        return values[0], values[0]
+    rng = random.Random(seed)
+    reps = sorted(_median([values[rng.randrange(len(values))] for _ in values]) for _ in range(iters))
+    return reps[int(alpha / 2 * iters)], reps[min(iters - 1, int((1 - alpha / 2) * iters))]
+
+
+def paired_budget(n: int):
+    """(rotating inputs, timed regions per block, repeats) by order
+    (scripts/_gpu_runner.py:2461-2473)."""
+    if n <= 128:
+        return 2, 20, 9
+    if n <= 512:
+        return 2, 15, 9
+    if n <= 2048:
+        return 2, 10, 7
+    if n <= 8192:
+        return 2, 5, 7
+    return 1, 3, 5
+
+
+def timed_block(fn, data_list, iters: int) -> float:
+    """Median us per call over ``iters`` cache-flushed, event-timed regions
+    (scripts/_gpu_runner.py:2476-2491)."""
+    dev = data_list[0].device
+    out = []
+    for _ in range(iters):
+        flush_caches(dev)
+        torch.cuda.synchronize(dev)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for d in data_list:
+            fn(d)
+        e1.record()
+        torch.cuda.synchronize(dev)
+        out.append(e0.elapsed_time(e1) * 1e3 / len(data_list))
+    return _median(out)
+
+
+def paired_compare(fn_a, fn_b, data_list, iters: int, repeats: int, warmup: int = 2) -> dict:
+    """Same-process A-B-B-A comparison of two callables on the same inputs
+    (scripts/_gpu_runner.py:2494-2588): both are warmed first, then each
+    repeat times A, B, B, A blocks; the first repeat is discarded.  Reports
+    the median of the per-repeat ratios mean(A)/mean(B) (> 1: B faster) with
+    a bootstrap 95 % interval, and the A-vs-A spread inside a repeat (the
+    protocol's own noise floor)."""
+    for _ in range(warmup):
+        for d in data_list:
+            fn_a(d)
+        for d in data_list:
+            fn_b(d)
+    torch.cuda.synchronize(data_list[0].device)
+    ratios, a_us, b_us, spread = [], [], [], []
+    for rep in range(repeats + 1):
+        a1 = timed_block(fn_a, data_list, iters)
+        b1 = timed_block(fn_b, data_list, iters)
+        b2 = timed_block(fn_b, data_list, iters)
+        a2 = timed_block(fn_a, data_list, iters)
+        if rep == 0:
+            continue
+        am, bm = 0.5 * (a1 + a2), 0.5 * (b1 + b2)
+        ratios.append(am / bm)
+        a_us.append(am)
+        b_us.append(bm)
+        spread.append(abs(a1 - a2) / am)
+    lo, hi = _bootstrap_ci(ratios)
+    return {"ratio_median": _median(ratios), "ratio_ci95": [lo, hi], "a_us": _median(a_us),
+            "b_us": _median(b_us), "order_spread_median": _median(spread), "repeats": len(ratios),
+            "iters_per_block": iters, "rotating_inputs": len(data_list)}
+
+
+# ---------------------------------------------------------------- stress ----
+# Adversarial inputs of the reference's stress grids, cheap at every n (no
+# QR): scripts/_gpu_runner.py:3051-3069 and :3374-3410.
+
+def adversarial_diagonal(batch: int, n: int, exponent: int, device=None):
+    """Diagonal SPD with eigenvalues logspace(0, -exponent, n); for exponent
+    10 the largest and smallest eigenvalues are interleaved (near-singular,
+    tiny diagonal and mixed dynamic range at once).  Returns (A, case name)
+    (scripts/_gpu_runner.py:3051-3069)."""
+    device = torch.device(device) if device is not None else _default_device()
+    values = torch.logspace(0.0, -float(exponent), n, device=device, dtype=torch.float32)
+    case = f"high_kappa_1e{exponent}"
+    if exponent == 10:
+        order = torch.empty(n, device=device, dtype=torch.long)
+        half = (n + 1) // 2  # n // 2 for the even orders the reference uses
+        order[0::2] = torch.arange(half, device=device)
+        order[1::2] = torch.arange(n - 1, half - 1, -1, device=device)
+        values = values[order]
+        case = "mixed_dynamic_near_singular_1e10"
+    return torch.diag_embed(values.unsqueeze(0).expand(batch, -1)).contiguous(), case
+
+
+def mixed_dynamic_spd(batch: int, n: int, cond: int, seed: int, device=None):
+    """D T D with T tridiagonal (diagonal U(1.75, 2.25), off-diagonal
+    U(-0.08, 0.08)) and D = diag(logspace(0, -cond/2, n)): SPD with ~10^cond
+    dynamic range (scripts/_gpu_runner.py:3374-3392)."""
+    device = torch.device(device) if device is not None else _default_device()
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    d = torch.empty((batch, n), device=device, dtype=torch.float32).uniform_(1.75, 2.25, generator=gen)
+    off = torch.empty((batch, max(n - 1, 0)), device=device, dtype=torch.float32).uniform_(-0.08, 0.08, generator=gen)
+    base = torch.diag_embed(d)
+    if n > 1:
+        base = base + torch.diag_embed(off, offset=1) + torch.diag_embed(off, offset=-1)
+    scales = torch.logspace(0.0, -0.5 * float(cond), n, device=device, dtype=torch.float32)
+    return (scales.reshape(1, n, 1) * base * scales.reshape(1, 1, n)).contiguous()
+
+
+def near_singular_banded(batch: int, n: int, cond: int, device=None):
+    """Diagonal logspace(0, -cond, n) with every other neighbouring pair
+    coupled by 0.125 sqrt(d_i d_{i+1}): SPD 2x2 blocks down to 10^-cond
+    (scripts/_gpu_runner.py:3395-3410)."""
+    device = torch.device(device) if device is not None else _default_device()
+    values = torch.logspace(0.0, -float(cond), n, device=device, dtype=torch.float32)
+    out = torch.diag_embed(values.expand(batch, n)).contiguous()
+    if n > 1:
+        pair = torch.arange(n - 1, device=device)
+        coupled = (pair % 2) == 0
+        off = torch.zeros(n - 1, device=device, dtype=torch.float32)
+        off[coupled] = 0.125 * torch.sqrt(values[:-1][coupled] * values[1:][coupled])
+        out = out + torch.diag_embed(off.expand(batch, n - 1), offset=1)
+        out = out + torch.diag_embed(off.expand(batch, n - 1), offset=-1)
+    return out.contiguous()
+
+
+def stress_cases(batch: int, n: int, seed: int, device=None):
+    """The adversarial set of one shape, as (name, A) pairs: for cond 6, 8
+    and 10 the tiny-diagonal family, the near-singular banded and the D T D
+    inputs (scripts/_gpu_runner.py:3454-3467), and the high-kappa diagonals
+    1e6 / 1e8 / interleaved 1e10 (:3051-3069)."""
+    for cond in (6, 8, 10):
+        yield f"tiny_diagonal_1e{cond}", generate_input(batch, n, cond, seed + 7100 + cond, "diagonal", device)
+        yield f"near_singular_banded_1e{cond}", near_singular_banded(batch, n, cond, device)
+        yield f"mixed_dynamic_1e{cond}", mixed_dynamic_spd(batch, n, cond, seed + 9100 + cond, device)
+    for e in (6, 8, 10):
+        a, name = adversarial_diagonal(batch, n, e, device)
+        yield name, a

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define CHOL_ABI_VERSION 1
+#define CHOL_ABI_VERSION 2
 
 /* Error codes (negative; positive values are hipError_t). */
 #define CHOL_E_ARG (-1)       /* bad size, stride or null pointer            */
@@ -43,6 +43,12 @@ extern "C" {
 /* Precision modes. */
 #define CHOL_MODE_PLAIN 0 /* all-FP32 arithmetic                            */
 #define CHOL_MODE_MIXED 1 /* BF16x3 split-product panels and trailing updates, FP32 acc */
+
+/* Routes of chol_potrf_f32 (chol_route). */
+#define CHOL_ROUTE_NONE 0    /* batch == 0 or n == 0: nothing launched        */
+#define CHOL_ROUTE_SMALL 1   /* n <= 64: whole matrices in wavefront registers */
+#define CHOL_ROUTE_LDS 2     /* 64 < n <= 256: whole matrix in one LDS         */
+#define CHOL_ROUTE_BLOCKED 3 /* blocked path, outer blocks of nbo columns      */
 
 /* Version of this ABI (CHOL_ABI_VERSION). */
 int chol_abi_version(void);
@@ -80,8 +86,49 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n,
                    int32_t* info, void* ws, size_t ws_bytes, void* stream);
 
 /*
- * Route / blocking table written by the autotuner: `rows` rows of 5 int64
- *   {n_lo, n_hi, batch_lo, batch_hi, nbo}   (inclusive ranges)
+ * The route chol_potrf_f32 takes for (batch, n, mode) under the current
+ * tuning table (CHOL_ROUTE_*), and through nbo_out (nullable) the outer
+ * block width of the blocked route (0 otherwise).  Host only, no HIP call.
+ * Replaces: the (batch, n) tables of the reference's dispatcher, read
+ * inline in custom_kernel (submission.py:4101-4268).
+ */
+int chol_route(int64_t batch, int64_t n, int32_t mode, int64_t* nbo_out);
+
+/*
+ * Dry run of the blocked route: the launches and stream-ordering steps that
+ * chol_potrf_f32 would enqueue for one batch of at most 65535 matrices of
+ * order n with leading dimension ld, on a device with `cus` compute units
+ * (<= 0: 256), without any HIP call.  *count receives the number of
+ * records; the first min(*count, cap) are written to rec, 10 int64 each:
+ *   {kind, stream (0 = caller's, 1 = side), fields...}
+ *   kind 1 diagonal block : k0, first-touch, carried update c0, c1, k0,
+ *                           flags (first-touch | in-kernel << 1), zero-fill
+ *                           rows r0, r1 (carried / zero fields -1 if none)
+ *   kind 2 panel          : k0, first-touch
+ *   kind 3 trailing update: col0, col1, k0, kl, first-touch
+ *   kind 4 zero fill of L's upper triangle; kind 5 side stream waits for
+ *   the caller; kind 6 mark on the side stream; kind 7 caller waits for
+ *   the last mark.
+ * Other routes give *count = 0.  Replaces nothing in the reference; it
+ * exposes the schedule so it can be checked on a host without a GPU.
+ */
+int chol_plan_potrf(int64_t batch, int64_t n, int64_t ld, int32_t mode,
+                    int32_t cus, int64_t* rec, int64_t cap, int64_t* count);
+
+/*
+ * Successful calls per route since the library was loaded, in the order
+ * {small, LDS, blocked, ragged, backward, triangular inverse}: the first
+ * min(cap, 6) are written to out; returns 6.  Replaces the per-route
+ * module counters of the reference (_CUDA32_HITS etc., submission.py:92,
+ * :179-182, :194, :302, :620), which its runner collects
+ * (scripts/_gpu_runner.py:2402-2414); there are no fallbacks here.
+ */
+int64_t chol_route_counters(int64_t* out, int64_t cap);
+
+/*
+ * Route / blocking table written by the autotuner: `rows` rows of 6 int64
+ *   {n_lo, n_hi, batch_lo, batch_hi, mode, nbo}   (inclusive ranges)
+ * mode = CHOL_MODE_PLAIN or CHOL_MODE_MIXED, or -1 for both forms;
  * nbo = outer block width of the blocked path (a multiple of 128, <= 4096),
  * or -1 to keep the whole-matrix LDS kernel (only for n_hi <= 256).  The
  * first matching row wins; unmatched shapes use built-in defaults; rows = 0

@@ -14,6 +14,13 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
   trailing update uses a chosen product (reference pattern
   scripts/_gpu_runner.py:1710-1762), to measure what each precision does to
   the reconstruction residual before any GPU time is spent.
+* ``correction_step``: one first-order refinement of a factor in FP64,
+  L + L Phi(L^-1 (A - L L^T) L^-T), the step a low-precision factor would
+  need to win its accuracy back.
+
+Each function is pinned in tests/test_oracle_emulate.py (torch's BF16 and
+FP8 casts, exact rational arithmetic, per-tile brute force, the FP64
+oracle, quadratic convergence, planted mistakes).
 """
 from __future__ import annotations
 
@@ -61,7 +68,6 @@ def e4m3_tile_nt(a: np.ndarray, b: np.ndarray, tile: int = 128) -> np.ndarray:
     (scale = 448 / absmax of the block), products accumulated in FP32."""
     def q(m):
         out = np.empty_like(m, dtype=np.float32)
-        sc = np.ones(((m.shape[0] + tile - 1) // tile, (m.shape[1] + tile - 1) // tile))
         for i in range(0, m.shape[0], tile):
             for j in range(0, m.shape[1], tile):
                 blk = m[i:i + tile, j:j + tile]
@@ -102,3 +108,25 @@ def blocked_cholesky(a: np.ndarray, nb: int = 64, product: str = "fp32") -> np.n
             raise ValueError(product)
         A[k + kb:, k + kb:] -= upd
     return np.tril(A)
+
+
+def phi(m: np.ndarray) -> np.ndarray:
+    """Lower triangle of ``m`` with its diagonal halved."""
+    p = np.tril(m)
+    idx = np.arange(m.shape[-1])
+    p[..., idx, idx] *= 0.5
+    return p
+
+
+def correction_step(a: np.ndarray, L: np.ndarray) -> np.ndarray:
+    """One first-order correction of a Cholesky factor, all in FP64:
+    with M = L^-1 (A - L L^T) L^-T, (I + X)(I + X)^T = I + M is solved to
+    first order by the lower-triangular X = Phi(M), so L <- L + L Phi(M).
+    The remaining error is second order in M."""
+    A = np.asarray(a, dtype=np.float64)
+    A = np.tril(A) + np.tril(A, -1).T
+    L = np.asarray(L, dtype=np.float64)
+    E = A - L @ L.T
+    Y = np.linalg.solve(L, E)          # L^-1 E
+    M = np.linalg.solve(L, Y.T).T      # (L^-1 E) L^-T
+    return L + L @ phi(0.5 * (M + M.T))

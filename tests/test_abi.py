@@ -30,7 +30,20 @@ def test_header_symbols_exported():
 
 
 def test_abi_version():
-    assert _lib.load().chol_abi_version() == 1
+    assert _lib.load().chol_abi_version() == 2
+
+
+def test_tuning_table_rejects_bad_rows():
+    import numpy as np
+    lib = _lib.load()
+    bad_mode = np.array([[300, 300, 1, 8, 5, 256]], dtype=np.int64)
+    assert lib.chol_set_tuning(bad_mode.ctypes.data, 1) == _lib.E_MODE
+    bad_nbo = np.array([[300, 300, 1, 8, 0, 200]], dtype=np.int64)
+    assert lib.chol_set_tuning(bad_nbo.ctypes.data, 1) == _lib.E_ARG
+    lds_big = np.array([[300, 300, 1, 8, -1, -1]], dtype=np.int64)
+    assert lib.chol_set_tuning(lds_big.ctypes.data, 1) == _lib.E_ARG
+    from gpumode_cholesky_amd import autotune
+    autotune.load_table()
 
 
 FAKE = ctypes.c_void_p(0x1000)  # never dereferenced: every call below fails validation
@@ -79,13 +92,21 @@ def test_rejects_short_workspace():
 def test_tuning_table_validation():
     import numpy as np
     lib = _lib.load()
-    ok = np.array([[65, 256, 1, 10**9, -1], [257, 4096, 1, 8, 256]], dtype=np.int64)
-    assert lib.chol_set_tuning(ok.ctypes.data, 2) == 0
-    bad = np.array([[300, 512, 1, 10, -1]], dtype=np.int64)  # LDS route beyond n = 256
+    # rows {n_lo, n_hi, batch_lo, batch_hi, mode, nbo}
+    ok = np.array([[65, 256, 1, 10**9, -1, -1], [257, 4096, 1, 8, 0, 256], [257, 4096, 1, 8, 1, 512]],
+                  dtype=np.int64)
+    assert lib.chol_set_tuning(ok.ctypes.data, 3) == 0
+    nbo = ctypes.c_int64(0)
+    assert lib.chol_route(4, 300, 0, ctypes.byref(nbo)) == 3 and nbo.value == 256
+    assert lib.chol_route(4, 300, 1, ctypes.byref(nbo)) == 3 and nbo.value == 512
+    assert lib.chol_route(4, 200, 1, ctypes.byref(nbo)) == 2
+    bad = np.array([[300, 512, 1, 10, -1, -1]], dtype=np.int64)  # LDS route beyond n = 256
     assert lib.chol_set_tuning(bad.ctypes.data, 1) == _lib.E_ARG
-    bad = np.array([[300, 512, 1, 10, 100]], dtype=np.int64)  # not a multiple of 64
+    bad = np.array([[300, 512, 1, 10, 0, 100]], dtype=np.int64)  # not a multiple of 128
     assert lib.chol_set_tuning(bad.ctypes.data, 1) == _lib.E_ARG
     assert lib.chol_set_tuning(None, 0) == 0
+    from gpumode_cholesky_amd import autotune
+    autotune.load_table()
 
 
 def test_custom_kernel_refuses_cpu_tensors():

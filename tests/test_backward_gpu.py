@@ -100,6 +100,36 @@ def test_backward_finite_differences(oracle_mod):
     assert worst <= BWD_TOL
 
 
+def test_backward_finite_differences_8x2048():
+    # Config: 8 x 2048, spectral cond 1e2; all 8 matrices, central FD in FP64
+    # (h = 1e-6) on the host along two random symmetric directions each.
+    # Relative error of <Abar, E> against the FD quotient <= BWD_TOL.
+    a = harness.spectral_input(8, 2048, 1e2, 2048, device=DEV)
+    gL = torch.tril(torch.randn(8, 2048, 2048, device=DEV, generator=torch.Generator(device=DEV).manual_seed(9)))
+    A = a.clone().requires_grad_(True)
+    L = cholesky_ad(A)
+    (L * gL).sum().backward()
+    g = A.grad.cpu().double().numpy()
+    a_h = a.cpu().double().numpy()
+    w_h = gL.cpu().double().numpy()
+    del A, L, gL
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    worst = 0.0
+    for b in range(8):
+        for _ in range(2):
+            E = rng.standard_normal((2048, 2048))
+            E = 0.5 * (E + E.T)
+            fp = (w_h[b] * np.linalg.cholesky(a_h[b] + h * E)).sum()
+            fm = (w_h[b] * np.linalg.cholesky(a_h[b] - h * E)).sum()
+            fd = (fp - fm) / (2 * h)
+            an = (g[b] * E).sum()
+            rel = abs(an - fd) / max(abs(fd), 1e-12)
+            print(f"[backward-fd-2048] matrix {b}: <Abar,E>={an:.6e} fd={fd:.6e} rel={rel:.3e}", flush=True)
+            worst = max(worst, rel)
+    assert worst <= BWD_TOL
+
+
 def test_backward_large_config():
     # 8 x 2048, cond 1e2: two sampled matrices against torch fp64 autograd.
     a = harness.spectral_input(8, 2048, 1e2, 77, device=DEV)
