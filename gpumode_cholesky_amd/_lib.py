@@ -40,6 +40,7 @@ SIGNATURES = {
     "chol_route": (ctypes.c_int, [_i64, _i64, _i32, _p]),
     "chol_plan_potrf": (ctypes.c_int, [_i64, _i64, _i64, _i32, _i32, _p, _i64, _p]),
     "chol_route_counters": (_i64, [_p, _i64]),
+    "chol_potrf_f32_noalloc": (ctypes.c_int, [_p, _p, _i64, _i64, _i64, _i64, _i32, _p, _p]),
 }
 
 

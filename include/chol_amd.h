@@ -86,6 +86,24 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n,
                    int32_t* info, void* ws, size_t ws_bytes, void* stream);
 
 /*
+ * chol_potrf_f32 without any allocation: no caller workspace, no hipMalloc,
+ * no stream or event creation; every launch on `stream`.  The blocked
+ * route's workspace (64 KiB per matrix of order > 64) comes from a device
+ * array the library holds, CHOL_NOALLOC_SCRATCH_BYTES per device until the
+ * process ends, so a call takes at most 1024 such matrices (more returns
+ * CHOL_E_WORKSPACE, nothing launched); orders <= 64 have no limit.  Calls
+ * that overlap on one device share that array and race: serialise them.
+ * Same arguments and results as chol_potrf_f32 (bit for bit), without the
+ * side-stream overlap.  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).
+ */
+#define CHOL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int chol_potrf_f32_noalloc(const float* A, float* L, int64_t batch, int64_t n,
+                           int64_t batch_stride, int64_t ld, int32_t mode,
+                           int32_t* info, void* stream);
+
+/*
  * The route chol_potrf_f32 takes for (batch, n, mode) under the current
  * tuning table (CHOL_ROUTE_*), and through nbo_out (nullable) the outer
  * block width of the blocked route (0 otherwise).  Host only, no HIP call.

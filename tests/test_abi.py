@@ -46,6 +46,14 @@ def test_tuning_table_rejects_bad_rows():
     autotune.load_table()
 
 
+def test_noalloc_rejects_over_limit_before_any_launch():
+    lib = _lib.load()
+    rc = lib.chol_potrf_f32_noalloc(FAKE, ctypes.c_void_p(0x2000), 1025, 300, 90000, 300, 0, None, None)
+    assert rc == _lib.E_WORKSPACE
+    assert "scratch" in _lib.last_error()
+    assert lib.chol_potrf_f32_noalloc(FAKE, FAKE, 0, 300, 0, 300, 0, None, None) == 0
+
+
 FAKE = ctypes.c_void_p(0x1000)  # never dereferenced: every call below fails validation
 
 
