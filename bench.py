@@ -180,6 +180,27 @@ def _time(specs, data, mode, steps, warmup, dev):
     return us
 
 
+def _copy_gbps(inputs, dev):
+    """A device copy of the same bytes as one factorisation of each input
+    (read A, write an output of the same size), timed like the shapes: the
+    practical ceiling of a kernel that reads and writes each byte once."""
+    outs = [torch.empty_like(a) for a in inputs]
+    ts = []
+    for _ in range(6):
+        torch.cuda.synchronize()
+        harness.flush_caches(dev)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for a, o in zip(inputs, outs):
+            o.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e3 / len(inputs))
+    ts.sort()
+    return 2.0 * inputs[0].numel() * 4 / ts[len(ts) // 2] / 1e3
+
+
 def _event_us(fn, reps=5):
     """Median microseconds of fn() over reps event-timed calls (cache flushed)."""
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -221,15 +242,20 @@ def extras(dev):
     torch.cuda.empty_cache()
     print("# extras: ragged done", file=sys.stderr, flush=True)
 
-    # Graph replay vs eager calls at the two smallest fixed shapes.
+    # Graph replay vs eager calls: the two smallest fixed shapes, and calls
+    # that leave most CUs idle; one graph of 8 sequential calls, and one of
+    # 8 independent branches.
     gr = {}
-    for b, n in ((4096, 32), (1024, 64)):
+    for b, n in ((4096, 32), (1024, 64), (16, 128), (4, 256)):
         ins = [harness.config_input(b, n, 31 + i, device=dev) for i in range(8)]
         g = GraphedCholesky(ins)
-        eager = _event_us(lambda: [cholesky(a) for a in ins], 5) / len(ins)
-        replay = _event_us(g.replay, 5) / len(ins)
-        gr[f"{b}x{n}"] = {"eager_us": round(eager, 2), "graph_us": round(replay, 2)}
-        del g, ins
+        gb = GraphedCholesky(ins, branches=True)
+        eager = _event_us(lambda: [cholesky(a) for a in ins], 9) / len(ins)
+        replay = _event_us(g.replay, 9) / len(ins)
+        replay_b = _event_us(gb.replay, 9) / len(ins)
+        gr[f"{b}x{n}"] = {"eager_us": round(eager, 2), "graph_us": round(replay, 2),
+                          "graph_branches_us": round(replay_b, 2)}
+        del g, gb, ins
     out["graph"] = gr
     print("# extras: graph done", file=sys.stderr, flush=True)
 
@@ -337,18 +363,20 @@ def main():
     ms_per_step = sum(times[head].values()) / 1e3
 
     # Headline roofline: the single-launch wave-register kernel on 4096x32
-    # (one launch per call, so the per-call event time is its duration).
+    # (one launch per call, so the per-call event time is its duration; both
+    # forms run this same kernel, so the faster of the two per-call means).
     roof = None
     r32 = next((r for r in rows if (r["batch"], r["n"]) == (4096, 32)), None)
     if r32 is not None:
-        g = r32[head]["GBps"]
+        g = max(r32[m]["GBps"] for m in modes)
         # traffic: HBM bytes per launch from the two rocprofv3 --pmc passes
         # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scaled by a 256 MiB copy
         # probe), recorded in profiles/k_potrf_small_32_4096x32.md.
         roof = {"kernel": "k_potrf_small<32,vec> on 4096x32", "bound": "hbm",
                 "achieved": g, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(g / HBM_PEAK_GBS, 4), "traffic": PMC_TRAFFIC_4096x32,
-                "algorithmic_bytes": 8 * 4096 * 32 * 32}
+                "algorithmic_bytes": 8 * 4096 * 32 * 32,
+                "copy_same_bytes_GBps": round(_copy_gbps(data[(4096, 32)], dev), 1)}
     cpu = fwd = None
     if not args.no_cpu_baseline:
         cpu, fwd = cpu_baseline(specs, data, host_L)

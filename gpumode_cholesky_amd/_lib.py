@@ -32,7 +32,7 @@ SIGNATURES = {
     "chol_potrf_f32": (ctypes.c_int, [_p, _p, _i64, _i64, _i64, _i64, _i32, _p, _p, _sz, _p]),
     "chol_set_tuning": (ctypes.c_int, [_p, _i64]),
     "chol_ragged_workspace_bytes": (_sz, [_p, _i64]),
-    "chol_potrf_ragged_f32": (ctypes.c_int, [_p, _p, _p, _i64, _i32, _p, _p, _sz, _p]),
+    "chol_potrf_ragged_f32": (ctypes.c_int, [_p, _p, _p, _p, _i64, _i32, _p, _p, _sz, _p]),
     "chol_backward_workspace_bytes": (_sz, [_i64, _i64]),
     "chol_backward_f32": (ctypes.c_int, [_p, _p, _p, _i64, _i64, _p, _i32, _p, _sz, _p]),
     "chol_tri_inverse_workspace_bytes": (_sz, [_i64, _i64]),

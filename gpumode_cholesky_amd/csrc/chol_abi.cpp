@@ -74,6 +74,14 @@ static size_t potrf_ws(int64_t batch, int64_t n) {
 
 size_t potrf_workspace(int64_t batch, int64_t n) { return potrf_ws(batch, n); }
 
+int64_t tuned_outer_width(int64_t n, int64_t batch, int32_t mode) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (const TuneRow& r : g_tune)
+    if (n >= r.n_lo && n <= r.n_hi && batch >= r.b_lo && batch <= r.b_hi && (r.mode < 0 || r.mode == mode))
+      return r.nbo;
+  return 0;
+}
+
 // Per-device side stream (default priority) and the two events that order
 // it against the caller's stream, created on first use and kept for the
 // life of the process.

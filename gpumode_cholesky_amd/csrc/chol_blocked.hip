@@ -31,22 +31,49 @@ namespace {
 // ---------------------------------------------------------------- copy ----
 // L = tril(A) with exact zeros above the diagonal.  Workgroup x takes rows
 // x, x + gridDim.x, ... of matrix blockIdx.y (any matrix of a ragged
-// table); its threads run along the row.  Where the source and destination
-// rows share their 16-byte alignment (both buffers 16-byte aligned and any
-// ld, or equal offsets), the row moves in float4 chunks between a scalar
-// head and tail of at most 3 elements each.
+// table); its threads run along the row.  A row of L that starts 16-byte
+// aligned (rows padded to a multiple of 4) is written in float4 chunks,
+// read from A as float4 where A's row is aligned too and as scalars where
+// not, the last n % 4 elements one by one.  Other rows move in float4
+// chunks between a scalar head and tail where A's and L's rows share their
+// 16-byte offset, element by element otherwise.
 __global__ __launch_bounds__(256) void k_copy_lower(Batch p) {
   const Mat M = mat_at(p, blockIdx.y);
   const int n = M.n;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    const float* src = M.A + (int64_t)i * M.ld;
+    const float* src = M.A + (int64_t)i * M.lda;
     float* dst = M.L + (int64_t)i * M.ld;
+    const int t = threadIdx.x;
+    if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+      const bool svec = (reinterpret_cast<uintptr_t>(src) & 15) == 0;
+      const int n4 = n >> 2;
+      for (int c = t; c < n4; c += blockDim.x) {
+        const int jj = 4 * c;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (jj <= i) {
+          if (svec) {
+            v = gld4_nt(src + jj);  // the chunk lies inside the row
+          } else {
+            v.x = gld_nt(src + jj);
+            if (jj + 1 <= i) v.y = gld_nt(src + jj + 1);
+            if (jj + 2 <= i) v.z = gld_nt(src + jj + 2);
+            if (jj + 3 <= i) v.w = gld_nt(src + jj + 3);
+          }
+        }
+        if (jj + 1 > i) v.y = 0.f;
+        if (jj + 2 > i) v.z = 0.f;
+        if (jj + 3 > i) v.w = 0.f;
+        gst4(dst + jj, v);
+      }
+      const int j = 4 * n4 + t;
+      if (t < (n & 3)) gst(dst + j, (j <= i) ? gld_nt(src + j) : 0.f);
+      continue;
+    }
     const int head = (int)((4 - ((reinterpret_cast<uintptr_t>(src) >> 2) & 3)) & 3);
     const bool vec = ((reinterpret_cast<uintptr_t>(src) ^ reinterpret_cast<uintptr_t>(dst)) & 15) == 0 && head < n;
     if (vec) {
       const int n4 = (n - head) >> 2;
       const int t0 = head + 4 * n4;  // first element of the tail
-      const int t = threadIdx.x;
       int j = -1;
       if (t < head)
         j = t;
@@ -63,7 +90,7 @@ __global__ __launch_bounds__(256) void k_copy_lower(Batch p) {
         gst4(dst + jj, v);
       }
     } else {
-      for (int jj = threadIdx.x; jj < n; jj += blockDim.x) gst(dst + jj, (jj <= i) ? gld_nt(src + jj) : 0.f);
+      for (int jj = t; jj < n; jj += blockDim.x) gst(dst + jj, (jj <= i) ? gld_nt(src + jj) : 0.f);
     }
   }
 }

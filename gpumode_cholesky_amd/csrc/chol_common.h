@@ -24,8 +24,9 @@ constexpr int kWave = 64;
 
 // Arguments of one batched factorisation (row-major, see chol_amd.h).
 // Dense form: matrix b at A + b*bs, order n, leading dimension ld.
-// Ragged form (Ap non-null): matrix b at Ap[b] / Lp[b], order ns[b],
-// contiguous (ld = ns[b]); its info slot is orig[b]; n is the largest order.
+// Ragged form (Ap non-null): matrix b at Ap[b] / Lp[b], order ns[b], A
+// contiguous (ld = ns[b]), L with leading dimension lds[b] (ns[b] when lds
+// is null); its info slot is orig[b]; n is the largest order.
 struct Batch {
   const float* A;
   float* L;
@@ -38,6 +39,7 @@ struct Batch {
   float* const* Lp = nullptr;
   const int32_t* ns = nullptr;
   const int32_t* orig = nullptr;
+  const int32_t* lds = nullptr;
   // Dense blocked path, first touch of a region: the kernel reads the old
   // values of its output region (C of a trailing update, A21 of a panel,
   // the diagonal block) from A instead of L, so no copy of tril(A) into L
@@ -50,8 +52,9 @@ struct Mat {
   const float* A;
   float* L;
   int n;
-  int64_t ld;
+  int64_t ld;    // leading dimension of L (and of A in the dense form)
   int64_t slot;  // index into info
+  int64_t lda;   // leading dimension of A
 };
 
 // Marks a pointer as global memory.  A pointer loaded from the ragged
@@ -94,9 +97,10 @@ __device__ __forceinline__ float gld_nt(const float* q) { return __builtin_nonte
 __device__ __forceinline__ Mat mat_at(const Batch& p, int64_t b) {
   if (p.Ap) {
     const int n = p.ns[b];
-    return Mat{as_global(p.Ap[b]), as_global(p.Lp[b]), n, (int64_t)n, p.orig ? (int64_t)p.orig[b] : b};
+    return Mat{as_global(p.Ap[b]), as_global(p.Lp[b]), n, p.lds ? (int64_t)p.lds[b] : (int64_t)n,
+               p.orig ? (int64_t)p.orig[b] : b, (int64_t)n};
   }
-  return Mat{as_global(p.A + b * p.bs), as_global(p.L + b * p.bs), p.n, p.ld, b};
+  return Mat{as_global(p.A + b * p.bs), as_global(p.L + b * p.bs), p.n, p.ld, b, p.ld};
 }
 
 // Compiler-level barrier between LDS traffic of lanes of one wavefront.
@@ -306,5 +310,8 @@ int potrf_dispatch(const char* who, const float* A, float* L, int64_t batch, int
                    int64_t ld, int32_t mode, int32_t* info, void* ws, size_t ws_bytes, hipStream_t s, bool side_ok);
 // Device workspace bytes of the dense dispatch (chol_workspace_bytes).
 size_t potrf_workspace(int64_t batch, int64_t n);
+// Outer block width of the tuning table for (n, batch, mode): a multiple of
+// kNB, -1 (the whole-matrix LDS route) or 0 when no row matches.
+int64_t tuned_outer_width(int64_t n, int64_t batch, int32_t mode);
 
 }  // namespace chol

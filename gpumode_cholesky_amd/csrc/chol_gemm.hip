@@ -249,7 +249,7 @@ template <int T, bool VEC, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void k_syrk_mfma(Batch p, int col0, int col1, int k0, int kl) {
   const int64_t b = blockIdx.y;
   const Mat M = mat_at(p, b);
-  const int c1 = col1 < 0 ? M.n : col1;  // col1 < 0: up to this matrix's order
+  const int c1 = (col1 < 0 || col1 > M.n) ? M.n : col1;  // col1 < 0: up to this matrix's order
   if (c1 <= col0) return;
   const int TI = (M.n - col0 + T - 1) / T;
   const int TJ = (c1 - col0 + T - 1) / T;

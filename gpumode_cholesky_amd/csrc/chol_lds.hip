@@ -60,7 +60,7 @@ __device__ __forceinline__ void lds_factor(const Batch& p, int64_t b) {
     for (int j = lane; j < len; j += 64) {
       float v;
       if (i < n)
-        v = (j <= i) ? gld_nt(A + (int64_t)i * M.ld + j) : 0.f;
+        v = (j <= i) ? gld_nt(A + (int64_t)i * M.lda + j) : 0.f;
       else
         v = (j == i) ? 1.f : 0.f;
       row[j] = v;

@@ -249,7 +249,7 @@ template <int T, bool VEC, bool SPLIT>
 __device__ __forceinline__ void syrk_once_tile(const Batch& p, int bid, int64_t b, int col0, int col1, int k0, bool first,
                                float* lds) {
   const Mat M = mat_at(p, b);
-  const int c1 = col1 < 0 ? M.n : col1;
+  const int c1 = (col1 < 0 || col1 > M.n) ? M.n : col1;
   if (c1 <= col0) return;
   const int TI = (M.n - col0 + T - 1) / T;
   const int TJ = (c1 - col0 + T - 1) / T;
@@ -285,7 +285,7 @@ template <int TM, int TN, bool VEC, bool SPLIT>
 __device__ __forceinline__ void syrk_once_rect(const Batch& p, int bid, int64_t b, int col0, int col1, int k0, bool first,
                                                float* lds, bool skip_diag) {
   const Mat M = mat_at(p, b);
-  const int c1 = col1 < 0 ? M.n : col1;
+  const int c1 = (col1 < 0 || col1 > M.n) ? M.n : col1;
   if (c1 <= col0) return;
   const int TJ = (c1 - col0 + TN - 1) / TN;
   int t = 0, rem = bid;

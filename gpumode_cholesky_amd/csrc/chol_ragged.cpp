@@ -2,21 +2,25 @@
 // orders factored together.
 //
 // The host sorts the matrices by order (largest first) and uploads one
-// descriptor table (pointers, orders, original positions) into the
-// workspace.  The sorted list splits into two size classes, each one
-// launch sequence over its contiguous slice:
+// descriptor table (pointers, orders, leading dimensions of L, original
+// positions) into the workspace.  The sorted list splits into two size
+// classes, each one launch sequence over its contiguous slice:
 //   n <= 64 : persistent wavefront kernel (k_potrf_small_queue), one
 //             launch per padded order 64/32/16, fed by an atomic work queue
 //             over the sorted slice;
-//   n > 64  : the blocked path run in lock step over 128-column steps: step
-//             k0 touches the matrices with n > k0, a prefix of the sorted
-//             list, and each kernel reads its matrix's order from the table
-//             (blocks past a smaller matrix exit).  Matrices of order
-//             65..256 join only the first one or two diagonal-block
-//             launches, whose workgroup count (one per CU) already takes
-//             two rounds when the larger class has more than 256 matrices;
-//             a whole-matrix LDS kernel per matrix measured ~3x slower per
-//             matrix at n = 128 and 256.
+//   n > 64  : L = tril(A) (k_copy_lower), then the blocked path in lock
+//             step: outer blocks of nbo columns (the tuning table's width
+//             for the largest order, else 256, 512 from 1024); inside one,
+//             128-column steps whose diagonal-block launch touches the
+//             matrices with n > k0 (a prefix of the sorted list) and whose
+//             panel and right-looking K = 128 update touch those with
+//             n > k0 + 128, the update clipped to the outer block and to
+//             each matrix's order; after it, one K = nbo update of every
+//             matrix's columns beyond the block.  Each kernel reads its
+//             matrix's order and leading dimension from the table (blocks
+//             past a smaller matrix exit).  Rows of L padded to a multiple
+//             of 4 (ld_host) give every kernel after the copy float4
+//             access whatever the orders.
 // The reference factors one fixed (batch, n) per call (custom_kernel,
 // submission.py:4101-4268); this generalises its per-shape dispatch.
 #include <algorithm>
@@ -31,7 +35,7 @@ namespace {
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-  size_t queues, Ap, Lp, ns, orig, winv, total;
+  size_t queues, Ap, Lp, ns, orig, lds, winv, total;
   int64_t nbig;
 };
 
@@ -44,7 +48,8 @@ Layout layout(const int64_t* n, int64_t count) {
   l.Lp = align256(l.Ap + count * sizeof(void*));
   l.ns = align256(l.Lp + count * sizeof(void*));
   l.orig = align256(l.ns + count * sizeof(int32_t));
-  l.winv = align256(l.orig + count * sizeof(int32_t));
+  l.lds = align256(l.orig + count * sizeof(int32_t));
+  l.winv = align256(l.lds + count * sizeof(int32_t));
   l.total = align256(l.winv + (size_t)nbig * kNB * kNB * sizeof(float));
   l.nbig = nbig;
   return l;
@@ -65,8 +70,9 @@ size_t chol_ragged_workspace_bytes(const int64_t* n_host, int64_t count) {
   return layout(n_host, count).total;
 }
 
-int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, const int64_t* n_host, int64_t count,
-                          int32_t mode, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
+int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, const int64_t* n_host,
+                          const int64_t* ld_host, int64_t count, int32_t mode, int32_t* info, void* ws,
+                          size_t ws_bytes, void* stream) {
   clear_error();
   if (count < 0 || (count > 0 && (!A_host || !L_host || !n_host))) {
     set_error("chol_potrf_ragged_f32: bad count or null host arrays");
@@ -86,6 +92,11 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
       set_error("chol_potrf_ragged_f32: matrix %lld: n=%lld or null pointer", (long long)i, (long long)n_host[i]);
       return CHOL_E_ARG;
     }
+    if (ld_host && (ld_host[i] < n_host[i] || ld_host[i] > (int64_t)1 << 30)) {
+      set_error("chol_potrf_ragged_f32: matrix %lld: ld=%lld < n=%lld", (long long)i, (long long)ld_host[i],
+                (long long)n_host[i]);
+      return CHOL_E_ARG;
+    }
   }
   const Layout lay = layout(n_host, count);
   if (!ws || ws_bytes < lay.total) {
@@ -103,6 +114,9 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
   auto* Lp = reinterpret_cast<float**>(desc.data() + (lay.Lp - lay.Ap));
   auto* ns = reinterpret_cast<int32_t*>(desc.data() + (lay.ns - lay.Ap));
   auto* orig = reinterpret_cast<int32_t*>(desc.data() + (lay.orig - lay.Ap));
+  auto* lds = reinterpret_cast<int32_t*>(desc.data() + (lay.lds - lay.Ap));
+  // float4 access by every kernel after the copy: each L row of a matrix
+  // with n > 64 starts 16-byte aligned (A is read by the copy alone)
   bool aligned = true;
   for (int64_t i = 0; i < count; ++i) {
     const int64_t j = ord[i];
@@ -110,8 +124,9 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
     Lp[i] = L_host[j];
     ns[i] = (int32_t)n_host[j];
     orig[i] = (int32_t)j;
-    aligned = aligned && (ns[i] % 4 == 0) && ((reinterpret_cast<uintptr_t>(A_host[j]) |
-                                                reinterpret_cast<uintptr_t>(L_host[j])) & 15) == 0;
+    lds[i] = (int32_t)(ld_host ? ld_host[j] : n_host[j]);
+    if (ns[i] > 64)
+      aligned = aligned && (lds[i] % 4 == 0) && (reinterpret_cast<uintptr_t>(L_host[j]) & 15) == 0;
   }
   char* w = reinterpret_cast<char*>(ws);
   // Pageable source: the copy is staged before hipMemcpyAsync returns, so
@@ -131,6 +146,7 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
   base.Lp = reinterpret_cast<float* const*>(w + lay.Lp);
   base.ns = reinterpret_cast<const int32_t*>(w + lay.ns);
   base.orig = reinterpret_cast<const int32_t*>(w + lay.orig);
+  base.lds = reinterpret_cast<const int32_t*>(w + lay.lds);
 
   // Class boundaries in the sorted list (orders decrease along it).
   auto first_at_most = [&](int64_t lim) {  // first sorted index with n <= lim
@@ -142,6 +158,7 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
 
   // n > 64: lock-step blocked path over the prefix [0, c64), in chunks of at
   // most 65535 matrices (the kernels put the matrix on grid.y).
+  const bool split = mode == CHOL_MODE_MIXED;
   for (int64_t c0 = 0; c0 < c64; c0 += 65535) {
     const int64_t cnt = std::min<int64_t>(c64 - c0, 65535);
     const int32_t* cns = ns + c0;  // host copy of the chunk's orders (sorted, largest first)
@@ -150,20 +167,33 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
     q.Lp = base.Lp + c0;
     q.ns = base.ns + c0;
     q.orig = base.orig + c0;
+    q.lds = base.lds + c0;
     q.batch = cnt;
     q.n = cns[0];
     float* winv = reinterpret_cast<float*>(w + lay.winv) + c0 * kNB * kNB;
     if ((rc = launch_copy_lower(q, s))) return rc;
-    for (int k0 = 0; k0 < cns[0]; k0 += kNB) {
-      int64_t act = 0, act2 = 0;
-      while (act < cnt && cns[act] > k0) ++act;
-      while (act2 < cnt && cns[act2] > k0 + kNB) ++act2;
-      q.batch = act;
-      if ((rc = launch_diag_factor(q, k0, winv, s))) return rc;
-      if (act2 == 0) break;
-      q.batch = act2;
-      if ((rc = launch_panel(q, k0, winv, mode == CHOL_MODE_MIXED, s))) return rc;
-      if ((rc = launch_syrk(q, k0 + kNB, -1, k0, kNB, mode == CHOL_MODE_MIXED, s))) return rc;
+    auto active = [&](int64_t above) {  // matrices of the chunk with n > above
+      int64_t a = 0;
+      while (a < cnt && cns[a] > above) ++a;
+      return a;
+    };
+    int64_t nbo = tuned_outer_width(cns[0], cnt, mode);
+    if (nbo < kNB || nbo % kNB != 0) nbo = cns[0] > 1024 ? 512 : 256;
+    for (int K0 = 0; K0 < cns[0]; K0 += (int)nbo) {
+      const int E = K0 + (int)nbo;  // end of the outer block
+      for (int k0 = K0; k0 < E && k0 < cns[0]; k0 += kNB) {
+        q.batch = active(k0);
+        if ((rc = launch_diag_factor(q, k0, winv, s))) return rc;
+        const int64_t act2 = active(k0 + kNB);
+        if (act2 == 0) break;
+        q.batch = act2;
+        if ((rc = launch_panel(q, k0, winv, split, s))) return rc;
+        if (k0 + kNB < E && (rc = launch_syrk(q, k0 + kNB, E, k0, kNB, split, s))) return rc;
+      }
+      const int64_t actE = active(E);
+      if (actE == 0) break;
+      q.batch = actE;
+      if ((rc = launch_syrk(q, E, -1, K0, (int)nbo, split, s))) return rc;
     }
   }
   // n <= 64: persistent wavefronts (4 per workgroup, 64/np matrices each).

@@ -31,7 +31,7 @@ template <int NP>
 __device__ __forceinline__ void slot_rows(const Batch& p, int64_t b, bool live, int r, float (&a)[NP]) {
   const Mat M = mat_at(p, live ? b : 0);
   const int n = M.n;
-  const float* src = M.A + (int64_t)(r < n ? r : 0) * M.ld;
+  const float* src = M.A + (int64_t)(r < n ? r : 0) * M.lda;
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
     float v;

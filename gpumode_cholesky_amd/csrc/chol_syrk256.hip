@@ -105,7 +105,7 @@ __global__ __launch_bounds__(NT) void k_syrk256(Batch p, int col0, int col1, int
 
   const int64_t b = blockIdx.y;
   const Mat M = mat_at(p, b);
-  const int c1 = col1 < 0 ? M.n : col1;
+  const int c1 = (col1 < 0 || col1 > M.n) ? M.n : col1;  // (ragged: clipped to this matrix)
   if (c1 <= col0) return;
   const int TI = (M.n - col0 + BT - 1) / BT;
   const int TJ = (c1 - col0 + BT - 1) / BT;

@@ -158,21 +158,25 @@ int64_t chol_route_counters(int64_t* out, int64_t cap);
 int chol_set_tuning(const int64_t* table, int64_t rows);
 
 /*
- * Ragged batch: `count` independent matrices of mixed orders n_host[i], each
- * contiguous (ld = n_host[i]).  A_host[i] / L_host[i] are DEVICE pointers to
- * matrix i; the three arrays themselves are in HOST memory.  info: nullable
+ * Ragged batch: `count` independent matrices of mixed orders n_host[i].
+ * A_host[i] / L_host[i] are DEVICE pointers to matrix i; A is contiguous
+ * (ld = n), L has leading dimension ld_host[i] (>= n; ld_host null: n).
+ * The three or four arrays themselves are in HOST memory.  info: nullable
  * device int32[count], in the caller's order.  The library sorts by order
  * and runs each size class as one launch sequence: a persistent wavefront
  * kernel (n <= 64) fed by atomic work queues over the size-sorted list, and
- * the blocked path in lock step over the matrices with n > 64.
+ * for n > 64 a copy of tril(A) into L followed by the blocked path in lock
+ * step over the matrices, with outer blocks of the tuning table's width
+ * (rows of L padded to a multiple of 4, 16-byte aligned, give every kernel
+ * after the copy float4 access).
  * Has no counterpart in the reference, which factors one fixed (batch, n)
  * per call; it generalises the per-shape dispatch of submission.py:4101-4268.
  */
 size_t chol_ragged_workspace_bytes(const int64_t* n_host, int64_t count);
 int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host,
-                          const int64_t* n_host, int64_t count, int32_t mode,
-                          int32_t* info, void* ws, size_t ws_bytes,
-                          void* stream);
+                          const int64_t* n_host, const int64_t* ld_host,
+                          int64_t count, int32_t mode, int32_t* info,
+                          void* ws, size_t ws_bytes, void* stream);
 
 /*
  * Adjoint of the factorisation: given L (from chol_potrf_f32) and the

@@ -120,8 +120,8 @@ def test_no_store_outside_buffers_ragged():
     info = Guarded(len(ns), torch.int32, SENT_I)
     for mode in (0, 1):
         print(f"[canary] ragged {ns} mode={mode}", flush=True)
-        rc = lib.chol_potrf_ragged_f32(a_host.ctypes.data, l_host.ctypes.data, n_host.ctypes.data, len(ns), mode,
-                                       info.ptr(), ws.ptr(), wsn, _stream())
+        rc = lib.chol_potrf_ragged_f32(a_host.ctypes.data, l_host.ctypes.data, n_host.ctypes.data, None, len(ns),
+                                       mode, info.ptr(), ws.ptr(), wsn, _stream())
         torch.cuda.synchronize()
         assert rc == 0, _lib.last_error()
         assert ws.margins_intact() and info.margins_intact()
@@ -154,3 +154,31 @@ def test_no_store_outside_buffers_inverse_and_backward(batch, n):
         torch.cuda.synchronize()
         assert rc == 0, _lib.last_error()
         assert G.margins_intact() and ws2.margins_intact() and X.margins_intact()
+
+
+def test_no_store_outside_buffers_ragged_padded_rows():
+    # L rows padded to a multiple of 4 (ld_host), as cholesky_ragged passes
+    # them: the padding columns [n, ld) of every row keep the sentinel.
+    lib = _lib.load()
+    ns = [301, 7, 1101, 64, 130, 66, 255, 1024, 97]
+    lds = [((n + 3) // 4 * 4 if n > 64 else n) for n in ns]
+    mats = [harness.generate_input(1, n, 2, 9300 + i, device=DEV)[0] for i, n in enumerate(ns)]
+    outs = [Guarded(n * ld, torch.float32, SENT_F) for n, ld in zip(ns, lds)]
+    n_host = np.array(ns, dtype=np.int64)
+    ld_host = np.array(lds, dtype=np.int64)
+    a_host = np.array([m.data_ptr() for m in mats], dtype=np.uint64)
+    l_host = np.array([o.ptr() for o in outs], dtype=np.uint64)
+    wsn = lib.chol_ragged_workspace_bytes(n_host.ctypes.data, len(ns))
+    ws = Guarded(wsn, torch.uint8, SENT_B)
+    for mode in (0, 1):
+        print(f"[canary] ragged padded {ns} mode={mode}", flush=True)
+        rc = lib.chol_potrf_ragged_f32(a_host.ctypes.data, l_host.ctypes.data, n_host.ctypes.data,
+                                       ld_host.ctypes.data, len(ns), mode, None, ws.ptr(), wsn, _stream())
+        torch.cuda.synchronize()
+        assert rc == 0, _lib.last_error()
+        assert ws.margins_intact()
+        for n, ld, o, a in zip(ns, lds, outs, mats):
+            assert o.margins_intact(), f"ragged n={n}: store outside its factor"
+            Lv = o.view.reshape(n, ld)
+            assert (Lv[:, n:] == SENT_F).all().item(), f"ragged n={n}: store into the row padding"
+            assert harness.fp64_residual(a[None], Lv[:, :n].contiguous()[None]) < 1e-5

@@ -16,14 +16,16 @@ pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda", 0)
 
 
+@pytest.mark.parametrize("branches", [False, True])
 @pytest.mark.parametrize("batch,n,mode", [(4096, 32, "plain"), (1024, 64, "plain"), (64, 256, "plain"),
                                           (4, 1024, "plain"), (4, 1024, "mixed"), (640, 512, "plain"),
-                                          (2, 2048, "mixed"), (1, 4096, "mixed"), (1, 4096, "plain")])
-def test_graph_replays_match_eager(batch, n, mode):
-    print(f"[graph] {batch}x{n} {mode}", flush=True)
+                                          (2, 2048, "mixed"), (1, 4096, "mixed"), (1, 4096, "plain"),
+                                          (4, 256, "mixed")])
+def test_graph_replays_match_eager(batch, n, mode, branches):
+    print(f"[graph] {batch}x{n} {mode} branches={branches}", flush=True)
     inputs = [harness.config_input(batch, n, 900 + i, device=DEV) for i in range(2)]
     eager = [cholesky(a, mode=mode) for a in inputs]
-    g = GraphedCholesky(inputs, mode=mode)
+    g = GraphedCholesky(inputs, mode=mode, branches=branches)
     for rep in range(3):
         outs = g.replay()
         torch.cuda.synchronize()
