@@ -229,9 +229,9 @@ def extras(dev):
 
     out = {}
     # Ragged: 512 matrices, n log-uniform on [32, 2048], each G G^T/n + I.
-    rng = np.random.default_rng(4096)
-    ns = np.exp(rng.uniform(math.log(32), math.log(2048), 512)).astype(int)
-    mats = [harness.dense_input(1, int(n), 1.0, 4096 + i, device=dev)[0] for i, n in enumerate(ns)]
+    from gpumode_cholesky_amd.ragged_workload import workload
+    mats = workload(dev)
+    ns = np.array([a.shape[-1] for a in mats])
     gb = sum(a.numel() * 4 for a in mats) / 1e9
     rag = {"count": len(mats), "gb": round(gb, 3), "n_min": int(ns.min()), "n_max": int(ns.max())}
     for mode in ("plain", "mixed"):

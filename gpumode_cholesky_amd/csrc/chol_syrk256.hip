@@ -21,12 +21,15 @@
 //
 // Tile order: workgroups are dispatched round-robin over the 8 XCDs, each
 // with its own L2; the block index is remapped so that every XCD takes a
-// contiguous run of the column-by-column tile list, whose neighbours share
-// the column operand block and nearby row blocks in that XCD's L2.
+// contiguous run of the tile list, and the list runs row by row through
+// groups of 4 tile columns (decode_tile_grouped), so the ~32 tiles an XCD
+// holds at once share 8 row blocks and 4 column blocks of the operand in
+// its L2.
 //
 // Reference counterpart: the trailing SYRK-as-GEMM updates of the blocked
 // drivers (submission.py:5163-5184, :2529-2534), run there on tensor cores.
 #include "chol_common.h"
+#include "chol_tiles.h"
 
 namespace chol {
 namespace {
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(NT) void k_syrk256(Batch p, int col0, int col1, int
   const int bid = blockIdx.x;
   if (bid >= T) return;
   int ti, tj;
-  decode_tile(xcd_tile(bid, T), TI, ti, tj);
+  decode_tile_grouped(xcd_tile(bid, T), TI, TJ, 4, ti, tj);
   const int i0 = col0 + BT * ti;
   const int j0 = col0 + BT * tj;
   float* L = M.L;
