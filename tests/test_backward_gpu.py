@@ -142,3 +142,28 @@ def test_backward_large_config():
         err = (A.grad[b].cpu().double() - ref).abs().max().item() / ref.abs().max().item()
         print(f"[backward-2048] matrix {b}: rel err {err:.3e}", flush=True)
         assert err <= BWD_TOL
+
+
+def test_inverse_and_backward_beyond_grid_limit():
+    # 32769 matrices of 256: the batched products of the inverse put
+    # matrices x diagonal pairs (and the backward's passes the matrices) on
+    # a grid dimension limited to 65535, so the launches split; checked on
+    # sampled matrices (both sides of each split) against FP64 on the host.
+    batch, n = 32769, 256
+    a = harness.spectral_input(8, n, 1e2, 4321, device=DEV)
+    idx = [0, 8191, 16383, 16384, 32767, 32768]
+    A = a[torch.arange(batch, device=DEV) % 8].contiguous()
+    del a
+    L = cholesky_ad(A)  # forward: blocked route for 32769 x 256
+    gL = torch.tril(torch.randn(batch, n, n, device=DEV, generator=torch.Generator(device=DEV).manual_seed(2)))
+    X = tri_inverse(L)
+    G = cholesky_backward(L, gL)
+    torch.cuda.synchronize()
+    for b in idx:
+        Lb = L[b].cpu().double()
+        Xr = torch.linalg.inv(Lb)
+        e_inv = (X[b].cpu().double() - Xr).abs().max().item() / Xr.abs().max().item()
+        ref = _torch_grad64(A[b].cpu(), gL[b].cpu())
+        e_bwd = (G[b].cpu().double() - ref).abs().max().item() / ref.abs().max().item()
+        print(f"[beyond-65535] matrix {b}: inverse {e_inv:.3e} backward {e_bwd:.3e}", flush=True)
+        assert e_inv < 1e-5 and e_bwd <= BWD_TOL

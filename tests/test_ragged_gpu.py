@@ -94,3 +94,23 @@ def test_ragged_workload_gate(mode):
     total = sum(a.numel() * 4 for a in mats) / 1e9
     print(f"[ragged-workload] {mode}: {len(mats)} matrices, {total:.2f} GB, worst fp64 resid {worst:.3e}", flush=True)
     assert worst <= (1e-5 if mode == "mixed" else 8 * math.sqrt(2048) * EPS32)
+
+
+def test_ragged_more_than_65535_large_matrices():
+    # 65600 matrices of orders 65..72 (all on the blocked lock-step path,
+    # whose kernels put the matrix on grid.y): the call splits into chunks of
+    # at most 65535; sampled factors on both sides of the split, and info.
+    count = 65600
+    ns = [65 + (i % 8) for i in range(count)]
+    base = {n: harness.generate_input(1, n, 2, 880 + n, device=DEV)[0] for n in set(ns)}
+    mats = [base[n] for n in ns]
+    outs, info = cholesky_ragged(mats, return_info=True)
+    torch.cuda.synchronize()
+    assert (info == 0).all().item()
+    for i in (0, 7, 65534, 65535, 65536, count - 1):
+        n = ns[i]
+        ref = cholesky(base[n][None])[0]
+        d = ((outs[i] - ref).abs().max() / ref.abs().max()).item()
+        r = harness.fp64_residual(base[n][None], outs[i].contiguous()[None])
+        print(f"[ragged>65535] matrix {i} n={n}: vs dense {d:.2e} resid {r:.2e}", flush=True)
+        assert d < 1e-5 and r <= 8 * math.sqrt(n) * EPS32, f"matrix {i} (n={n})"
