@@ -270,7 +270,12 @@ static int blocked_potrf(const Batch& p, float* winv, bool split, int nbo, hipSt
     // short launch each) when they are latency work.  These steps stay on
     // one stream: a cross-stream event pair costs ~10-20 us of idle time on
     // this stack, more than the microsecond-scale launches it could overlap.
-    const bool left = (int64_t)((n - K0) / kNB) * p.batch >= cus;
+    // From n = 24576 on every outer block is left-looking: there the
+    // right-looking K = 128 updates of the block's later columns are launches
+    // of thousands of short tiles that the side stream's bulk update
+    // starves of CUs (1 x 32768: -5 % mixed, -2 % plain; 8192 and 16384
+    // lose with it).
+    const bool left = (int64_t)((n - K0) / kNB) * p.batch >= cus || (n >= 24576 && (n - K0) > kNB);
     // first touch in the first outer block: its diagonal block and panel at
     // k0 = 0, the first update of each of its column blocks (right-looking:
     // the step-0 update of all of them; left-looking: each one's update),

@@ -236,7 +236,8 @@ def test_schedule_other_cu_counts(cus):
 
 
 @pytest.mark.parametrize("batch,n", [(4, 1024), (60, 1024), (2, 2048), (8, 2048), (1, 4096), (2, 4096),
-                                     (1, 8192), (1, 16384), (1, 32768), (16, 512), (640, 512)])
+                                     (1, 8192), (1, 16384), (1, 32768), (16, 512), (640, 512), (1, 24576),
+                                     (2, 24576), (1, 24700)])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_schedule_bench_shapes(batch, n, mode):
     rec = check(batch, n, n, mode)
