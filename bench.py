@@ -243,17 +243,18 @@ def extras(dev):
     print("# extras: ragged done", file=sys.stderr, flush=True)
 
     # Graph replay vs eager calls: the two smallest fixed shapes, and calls
-    # that leave most CUs idle; one graph of 8 sequential calls, and one of
-    # 8 independent branches.
+    # that leave most CUs idle.  Each shape's rotated inputs (as many as the
+    # timed protocol uses, min(50, 256 MiB / input)) in one graph of
+    # sequential calls, and in one of independent branches.
     gr = {}
     for b, n in ((4096, 32), (1024, 64), (16, 128), (4, 256)):
-        ins = [harness.config_input(b, n, 31 + i, device=dev) for i in range(8)]
+        ins = [harness.config_input(b, n, 31 + i, device=dev) for i in range(harness.rotation_count(b, n))]
         g = GraphedCholesky(ins)
         gb = GraphedCholesky(ins, branches=True)
         eager = _event_us(lambda: [cholesky(a) for a in ins], 9) / len(ins)
         replay = _event_us(g.replay, 9) / len(ins)
         replay_b = _event_us(gb.replay, 9) / len(ins)
-        gr[f"{b}x{n}"] = {"eager_us": round(eager, 2), "graph_us": round(replay, 2),
+        gr[f"{b}x{n}"] = {"inputs": len(ins), "eager_us": round(eager, 2), "graph_us": round(replay, 2),
                           "graph_branches_us": round(replay_b, 2)}
         del g, gb, ins
     out["graph"] = gr
