@@ -45,7 +45,7 @@ PMC_TRAFFIC_4096x32 = (8260.1 * 2.0 + 16384.0) * 1024  # bytes per launch
 def _parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="both", choices=["plain", "mixed", "both"])
     ap.add_argument("--shapes", default="", help="comma list like 4096x32,1x4096 (default: all 15)")
@@ -173,11 +173,12 @@ def _time(specs, data, mode, steps, warmup, dev):
             ev[key].append((e0, e1, len(data[key])))
         torch.cuda.synchronize()
         print(f"# {mode} step {step + 1}/{steps} done", file=sys.stderr, flush=True)
-    us = {}
+    us, per_step = {}, {}
     for key, lst in ev.items():
         vals = [e0.elapsed_time(e1) * 1e3 / c for (e0, e1, c) in lst]
         us[key] = sum(vals) / len(vals)
-    return us
+        per_step[key] = vals
+    return us, per_step
 
 
 def _copy_gbps(inputs, dev):
@@ -339,7 +340,8 @@ def main():
     resid = {m: _gate(specs, data, m, host_L) for m in modes}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    times = {m: _time(specs, data, m, args.steps, args.warmup, dev) for m in modes}
+    timed = {m: _time(specs, data, m, args.steps, args.warmup, dev) for m in modes}
+    times = {m: timed[m][0] for m in modes}
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
@@ -358,7 +360,8 @@ def main():
                       "TFLOPs": round(flops / us / 1e6, 3),
                       "hbm_frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 4),
                       "fp32_frac": round(flops / us / 1e6 / FP32_PEAK_TFLOPS, 4),
-                      "fp64_resid": resid[m][key]}
+                      "fp64_resid": resid[m][key],
+                      "step_us": [round(v, 2) for v in timed[m][1][key]]}
         rows.append(row)
     geo = {m: _geo(times[m][k] for k in times[m]) for m in modes}
     ms_per_step = sum(times[head].values()) / 1e3

@@ -30,18 +30,30 @@ def cholesky_ragged(mats, mode: str = "plain", return_info: bool = False, padded
     m = _MODES.get(mode)
     if m is None:
         raise ValueError(f"mode must be one of {sorted(_MODES)}, got {mode!r}")
-    mats = [a if a.is_contiguous() else a.contiguous() for a in mats]
     if not mats:
         return ([], torch.empty(0, dtype=torch.int32)) if return_info else []
+    _validate(mats[0])
     dev = mats[0].device
+    # One pass over the list with the cheapest checks (hundreds of matrices:
+    # the host time is part of the call); a failing matrix gets the full
+    # check for its message.
+    f32, idx = torch.float32, dev.index
+    ns, ptrs, keep = [], [], []
     for a in mats:
-        _validate(a)
-        if a.dim() != 2 or a.device != dev:
+        s = a.shape
+        if not isinstance(a, torch.Tensor) or a.dtype is not f32 or len(s) != 2 or s[0] != s[1] \
+                or a.get_device() != idx:
+            _validate(a)
             raise ValueError("every matrix must be a 2-D [n, n] tensor on one device")
+        if not a.is_contiguous():
+            a = a.contiguous()
+            keep.append(a)  # alive until the kernels have read it (same stream)
+        ns.append(s[0])
+        ptrs.append(a.data_ptr())
     count = len(mats)
-    n_host = np.array([a.shape[-1] for a in mats], dtype=np.int64)
+    n_host = np.array(ns, dtype=np.int64)
     ld_host = np.where(n_host > 64, (n_host + 3) // 4 * 4, n_host) if padded else n_host.copy()
-    a_host = np.array([a.data_ptr() for a in mats], dtype=np.uint64)
+    a_host = np.array(ptrs, dtype=np.uint64)
     # One output allocation instead of one per matrix (the host time of
     # hundreds of allocations is a visible share of a ragged call); the
     # views are made after the kernels are enqueued, while they run.
