@@ -12,7 +12,10 @@
 //   n >  256       : the blocked path: 128-wide diagonal blocks with explicit
 //                    inverses (chol_diag.hip), MFMA panels and SYRK
 //                    (chol_gemm.hip, chol_syrk256.hip), driven by
-//                    blocked_potrf below.
+//                    blocked_potrf below;
+//   any n > 64     : where the table says so, the fused kernel
+//                    (k_potrf_fused, chol_diag.hip): one workgroup factors
+//                    a whole matrix, for batches that fill the GPU.
 #include <array>
 #include <atomic>
 #include <cstdarg>
@@ -39,11 +42,14 @@ static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Largest order of the whole-matrix-in-LDS kernel.
 constexpr int64_t kLdsMaxN = 256;
+// Table values of nbo that name a route instead of an outer width.
+constexpr int64_t kRouteLds = -1, kRouteFused = -2;
 
 // Tuning table (chol_set_tuning): rows {n_lo, n_hi, batch_lo, batch_hi, mode,
 // nbo}; mode -1 matches both forms.  nbo > 0: blocked path with that outer
 // block width (a multiple of kNB = 128); nbo = -1: the whole-matrix LDS
-// kernel (n <= 256 only).  First match wins.
+// kernel (n <= 256 only); nbo = -2: the fused kernel (n > 64 only).  First
+// match wins.
 struct TuneRow {
   int64_t n_lo, n_hi, b_lo, b_hi, mode, nbo;
 };
@@ -373,10 +379,19 @@ int potrf_dispatch(const char* who, const float* A, float* L, int64_t batch, int
     return rc;
   }
   int64_t nbo = pick_nbo(n, batch, mode);
-  if (nbo < 0 && n <= kLdsMaxN) {
+  if (nbo == kRouteLds && n <= kLdsMaxN) {
     int rc = launch_lds(p, s);
     if (!rc) note_call(RC_LDS);
     return rc;
+  }
+  if (nbo == kRouteFused) {
+    int rc = launch_zero_info(info, batch, s);
+    if (rc) return rc;
+    Batch q = p;
+    q.first = true;  // A and L share ld: step 0 reads A where it first touches
+    if ((rc = launch_fused(q, reinterpret_cast<float*>(ws), mode == CHOL_MODE_MIXED, s))) return rc;
+    note_call(RC_FUSED);
+    return 0;
   }
   const int cus = device_cus();
   if (nbo < kNB || nbo % kNB != 0) nbo = 256;
@@ -434,8 +449,10 @@ int chol_route(int64_t batch, int64_t n, int32_t mode, int64_t* nbo_out) {
       route = CHOL_ROUTE_SMALL;
     } else {
       nbo = pick_nbo(n, batch, mode);
-      if (nbo < 0 && n <= kLdsMaxN) {
+      if (nbo == kRouteLds && n <= kLdsMaxN) {
         route = CHOL_ROUTE_LDS;
+      } else if (nbo == kRouteFused) {
+        route = CHOL_ROUTE_FUSED;
       } else {
         if (nbo < kNB || nbo % kNB != 0) nbo = 256;
         route = CHOL_ROUTE_BLOCKED;
@@ -488,9 +505,10 @@ int chol_set_tuning(const int64_t* table, int64_t rows) {
   for (int64_t i = 0; i < rows; ++i) {
     const int64_t* r = table + 6 * i;
     const int64_t mode = r[4], nbo = r[5];
-    if (!(nbo == -1 || (nbo >= kNB && nbo % kNB == 0 && nbo <= 4096)) || (nbo == -1 && r[1] > kLdsMaxN)) {
-      set_error("chol_set_tuning: row %lld: nbo=%lld must be -1 (n <= %lld) or a multiple of %d", (long long)i,
-                (long long)nbo, (long long)kLdsMaxN, kNB);
+    if (!(nbo == kRouteLds || nbo == kRouteFused || (nbo >= kNB && nbo % kNB == 0 && nbo <= 4096)) ||
+        (nbo == kRouteLds && r[1] > kLdsMaxN) || (nbo == kRouteFused && r[0] <= 64)) {
+      set_error("chol_set_tuning: row %lld: nbo=%lld must be -1 (n <= %lld), -2 (n > 64) or a multiple of %d",
+                (long long)i, (long long)nbo, (long long)kLdsMaxN, kNB);
       return CHOL_E_ARG;
     }
     if (mode != -1 && mode != CHOL_MODE_PLAIN && mode != CHOL_MODE_MIXED) {

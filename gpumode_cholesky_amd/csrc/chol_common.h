@@ -16,7 +16,7 @@ void set_error(const char* fmt, ...);
 void clear_error();
 
 // Successful calls per route, read by chol_route_counters (in its order).
-enum RouteCounter { RC_SMALL, RC_LDS, RC_BLOCKED, RC_RAGGED, RC_BACKWARD, RC_TRI_INVERSE, RC_COUNT };
+enum RouteCounter { RC_SMALL, RC_LDS, RC_BLOCKED, RC_RAGGED, RC_BACKWARD, RC_TRI_INVERSE, RC_FUSED, RC_COUNT };
 void note_call(RouteCounter r);
 
 // Wave size of CDNA: every kernel here is written for 64-lane wavefronts.
@@ -282,6 +282,10 @@ struct DiagZero {
 };
 int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const DiagBulk* bulk = nullptr,
                        const DiagZero* zero = nullptr);           // kNB x kNB block
+// Whole factorisation, one workgroup per matrix (k_potrf_fused): winv holds
+// kNB x kNB floats per matrix; p.first: A and L share ld and L is not yet
+// written (else L holds tril(A)); split: BF16x3 panel and update products.
+int launch_fused(const Batch& p, float* winv, bool split, hipStream_t s);
 int launch_panel(const Batch& p, int k0, const float* winv, bool split,
                  hipStream_t s);  // L21 = A21 W^T (split: BF16x3 products)
 // C -= P P^T over {col0 <= j < col1, j <= i < n}, K = [k0, k0+kl);

@@ -413,48 +413,34 @@ __device__ __forceinline__ void pre_tiles(float* Ls, const __bf16* Ph, int li, i
   }
 }
 
-// Workgroups past the batch: nz of them zero L's upper triangle in rows
-// [zr0, zr1); then, BULK (1 FP32, 2 BF16x3), the previous step's
-// trailing-update tiles (64x64, one-shot; bt per matrix).
-// PRE (with BULK == 2): the previous panel (columns bk0..bk0+127) has not
-// yet updated this block: the kernel applies that update to it first, and
-// the workgroups past the batch (skipping this block) the rest of it.
-template <bool FROM_A, bool LAST, int BULK, bool PRE = false>
-__global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv, int bc0, int bc1,
-                                                     int bk0, int bt, int bfirst, int zr0, int zr1, int nz) {
-  __shared__ DiagLds sm;
-  // Workgroups past the batch run on the CUs that the diagonal blocks leave
-  // idle: a share of the zero fill of L's upper triangle, and the previous
-  // step's trailing-update tiles beyond the next column block (which no
-  // diagonal block of this launch reads).
-  if (blockIdx.x >= p.batch) {
-    int id = blockIdx.x - (int)p.batch;
-    if (id < nz) {
-      zero_upper_rows(p, zr0, zr1, id, nz);
-      return;
-    }
-    id -= nz;
-    if constexpr (BULK != 0) {
-      syrk_once_rect<64, 128, true, BULK == 2>(p, id % bt, id / bt, bc0, bc1, bk0, bfirst != 0,
-                                               reinterpret_cast<float*>(&sm), PRE);
-    }
-    return;
-  }
+// The diagonal block at (k0, k0) of matrix b, one workgroup of 256 threads
+// (every thread calls it): L11 into L, its inverse W (unless LAST) into Wb
+// (128 x 128, row-major).  The body of k_diag_factor, and of each step of
+// the fused per-matrix kernel below.
+// PRE: the previous panel (columns bk0..bk0+127) has not yet updated this
+// block: the block's update is applied to it in LDS first (bfirst: that
+// panel is the first, the block's old values come from A).  from_a_rt: the
+// block is staged from A, as for FROM_A, chosen at run time.  opaque: the
+// thread index is hidden from the optimiser (see gemm_tile_once), for
+// callers that run this in a loop with other large code.
+template <bool FROM_A, bool LAST, bool PRE>
+__device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, float* __restrict__ Wb, DiagLds& sm,
+                                           int bk0, int bfirst, bool from_a_rt = false, bool opaque = false) {
   float(*const Ts)[16 * TP] = sm.Ts;
   float* const Ls = sm.Ls;
   float* const Ws = sm.Ws;
-  const int tid = threadIdx.x;
+  int tid = threadIdx.x;
+  if (opaque) asm volatile("" : "+v"(tid));
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15;
   const int lg = lane >> 4;
-  const int64_t b = blockIdx.x;
   const Mat M = mat_at(p, b);
   const int nb = min(NB, M.n - k0);
   if (nb <= 0) return;  // (ragged) this matrix has no block at k0
   float* Lb = M.L + (int64_t)k0 * M.ld + k0;
   // FROM_A: the first block, staged from A (nothing has written L yet).
-  const bool from_a = FROM_A || (PRE && bfirst != 0);
+  const bool from_a = FROM_A || (PRE && bfirst != 0) || from_a_rt;
   const float* Sb = from_a ? M.A + (int64_t)k0 * M.ld + k0 : Lb;
   const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0) &&
                    (!from_a || (reinterpret_cast<uintptr_t>(M.A) & 15) == 0);
@@ -627,7 +613,6 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   // columns.  The upper-right 64x64 quadrant of the block is stored (zeros)
   // only when the block is the whole matrix; otherwise the zero fill of L's
   // upper triangle covers it (every chunk right of a row's diagonal chunk).
-  float* Wb = winv + b * (NB * NB);
   auto store_l32 = [&](int r0, int c0) {
     float4 v[8];
 #pragma unroll
@@ -787,7 +772,132 @@ __global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __r
   }
 }
 
+// Workgroups past the batch: nz of them zero L's upper triangle in rows
+// [zr0, zr1); then, BULK (1 FP32, 2 BF16x3), the previous step's
+// trailing-update tiles (64x128, one-shot; bt per matrix).
+// PRE (with BULK == 2): the previous panel (columns bk0..bk0+127) has not
+// yet updated this block: the kernel applies that update to it first, and
+// the workgroups past the batch (skipping this block) the rest of it.
+template <bool FROM_A, bool LAST, int BULK, bool PRE = false>
+__global__ __launch_bounds__(256) void k_diag_factor(Batch p, int k0, float* __restrict__ winv, int bc0, int bc1,
+                                                     int bk0, int bt, int bfirst, int zr0, int zr1, int nz) {
+  __shared__ DiagLds sm;
+  // Workgroups past the batch run on the CUs that the diagonal blocks leave
+  // idle: a share of the zero fill of L's upper triangle, and the previous
+  // step's trailing-update tiles beyond the next column block (which no
+  // diagonal block of this launch reads).
+  if (blockIdx.x >= p.batch) {
+    int id = blockIdx.x - (int)p.batch;
+    if (id < nz) {
+      zero_upper_rows(p, zr0, zr1, id, nz);
+      return;
+    }
+    id -= nz;
+    if constexpr (BULK != 0) {
+      syrk_once_rect<64, 128, true, BULK == 2>(p, id % bt, id / bt, bc0, bc1, bk0, bfirst != 0,
+                                               reinterpret_cast<float*>(&sm), PRE);
+    }
+    return;
+  }
+  diag_block<FROM_A, LAST, PRE>(p, blockIdx.x, k0, winv + (int64_t)blockIdx.x * (NB * NB), sm, bk0, bfirst);
+}
+
+// ------------------------------------------------------------- fused ----
+// L = 0 in the whole 4-column chunks right of each row's diagonal chunk of
+// one matrix, by one workgroup (wavefront w: rows w, w+4, ...; lanes along
+// the row), streaming stores.  The chunks that hold a diagonal element are
+// stored by the diagonal-block code.
+__device__ __forceinline__ void zero_upper_one(const Mat& M) {
+  const int n = M.n;
+  const int n4 = (n + 3) >> 2;
+  const bool vec = ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0) && (M.ld & 3) == 0 && (n & 3) == 0;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  for (int i = wave; i < n; i += 4) {
+    float* row = M.L + (int64_t)i * M.ld;
+    for (int q = (i >> 2) + 1 + lane; q < n4; q += 64) {
+      const int c = 4 * q;
+      if (vec) {
+        gst4_nt(row + c, make_float4(0.f, 0.f, 0.f, 0.f));
+      } else {
+        for (int j = 0; j < 4 && c + j < n; ++j) gst_nt(row + c + j, 0.f);
+      }
+    }
+  }
+}
+
+// The whole factorisation of matrix b by one workgroup: for each 128
+// columns, the diagonal block and its inverse W (diag_block, W at Wb), the
+// panel L21 = A21 W^T in 64-row one-shot tiles, and the right-looking
+// update of the trailing lower triangle, C -= L21 L21^T, in 64 x 128
+// one-shot tiles (BF16x3 products when SPLIT); then the zeros above the
+// diagonal.  The matrix lives in L, where the caches keep the part the
+// workgroup works on; every tile is staged through LDS.  Phases are
+// separated by workgroup barriers only: all waves of a workgroup share one
+// CU and its L1, so after a barrier a wave's loads see the global stores
+// the others made before it.  p.first (dense, A and L with the same
+// leading dimension): step 0 reads the old values it first touches from A
+// (no copy of tril(A) into L); otherwise L already holds tril(A).
+template <bool VEC, bool SPLIT>
+__device__ __forceinline__ void fused_matrix(const Batch& p, int64_t b, float* __restrict__ Wb, DiagLds& sm) {
+  const Mat M = mat_at(p, b);
+  const int n = M.n;
+  float* const lds = reinterpret_cast<float*>(&sm);
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const bool first = p.first && k0 == 0;
+    __syncthreads();  // the previous step's tiles are done with the LDS
+    // One form of the block code for every step (a second inlined copy
+    // makes the kernel spill): the last block also forms its inverse,
+    // unused, and its upper right quadrant is left to zero_upper_one.
+    diag_block<false, false, false>(p, b, k0, Wb, sm, 0, 0, first, true);
+    if (k0 + NB >= n) break;
+    const float* As = first ? M.A : M.L;
+    for (int i0 = k0 + NB; i0 < n; i0 += 64) {
+      __syncthreads();  // W and L11 stored; the LDS free
+      gemm_tile_once<64, NB, 1, VEC, SPLIT>(As, M.ld, i0, n, k0, Wb, NB, 0, NB, 0, M.L, M.L, M.ld, i0, k0, n,
+                                            k0 + NB, lds, true);
+    }
+    const float* Cs = first ? M.A : M.L;
+    for (int j0 = k0 + NB; j0 < n; j0 += 128)
+      for (int i0 = j0; i0 < n; i0 += 64) {
+        __syncthreads();  // the panel stored; the LDS free
+        gemm_tile_once<64, 128, 0, VEC, SPLIT>(M.L, M.ld, i0, n, k0, M.L, M.ld, j0, n, k0, M.L, Cs, M.ld, i0, j0, n,
+                                               n, lds, true);
+      }
+  }
+  zero_upper_one(M);
+}
+
+// One workgroup per matrix (grid.x = the batch; workgroups are dispatched
+// in block order, so a ragged list sorted largest first starts its largest
+// matrices first).  W of matrix b at winv + 128 * 128 * b.
+template <bool VEC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_potrf_fused(Batch p, float* __restrict__ winv) {
+  __shared__ DiagLds sm;
+  const int64_t b = blockIdx.x;
+  fused_matrix<VEC, SPLIT>(p, b, winv + b * (NB * NB), sm);
+}
+
 }  // namespace
+
+int launch_fused(const Batch& p, float* winv, bool split, hipStream_t s) {
+  if (p.batch <= 0) return 0;
+  if (p.batch > 0x7fffffffLL) {
+    set_error("k_potrf_fused: batch %lld exceeds the grid", (long long)p.batch);
+    return CHOL_E_ARG;
+  }
+  const bool vec = vec_ok(p);
+  const dim3 g((unsigned)p.batch), b(256);
+  if (vec && split)
+    hipLaunchKernelGGL((k_potrf_fused<true, true>), g, b, 0, s, p, winv);
+  else if (vec)
+    hipLaunchKernelGGL((k_potrf_fused<true, false>), g, b, 0, s, p, winv);
+  else if (split)
+    hipLaunchKernelGGL((k_potrf_fused<false, true>), g, b, 0, s, p, winv);
+  else
+    hipLaunchKernelGGL((k_potrf_fused<false, false>), g, b, 0, s, p, winv);
+  return launch_status("k_potrf_fused");
+}
 
 int64_t diag_bulk_tiles(const Batch& p, const DiagBulk& u) {
   return (int64_t)rect_tiles<64, 128>(p.n, u.c0, u.c1) * p.batch;

@@ -49,6 +49,7 @@ extern "C" {
 #define CHOL_ROUTE_SMALL 1   /* n <= 64: whole matrices in wavefront registers */
 #define CHOL_ROUTE_LDS 2     /* 64 < n <= 256: whole matrix in one LDS         */
 #define CHOL_ROUTE_BLOCKED 3 /* blocked path, outer blocks of nbo columns      */
+#define CHOL_ROUTE_FUSED 4   /* n > 64: one workgroup factors a whole matrix  */
 
 /* Version of this ABI (CHOL_ABI_VERSION). */
 int chol_abi_version(void);
@@ -135,8 +136,8 @@ int chol_plan_potrf(int64_t batch, int64_t n, int64_t ld, int32_t mode,
 
 /*
  * Successful calls per route since the library was loaded, in the order
- * {small, LDS, blocked, ragged, backward, triangular inverse}: the first
- * min(cap, 6) are written to out; returns 6.  Replaces the per-route
+ * {small, LDS, blocked, ragged, backward, triangular inverse, fused}: the
+ * first min(cap, 7) are written to out; returns 7.  Replaces the per-route
  * module counters of the reference (_CUDA32_HITS etc., submission.py:92,
  * :179-182, :194, :302, :620), which its runner collects
  * (scripts/_gpu_runner.py:2402-2414); there are no fallbacks here.
@@ -148,7 +149,8 @@ int64_t chol_route_counters(int64_t* out, int64_t cap);
  *   {n_lo, n_hi, batch_lo, batch_hi, mode, nbo}   (inclusive ranges)
  * mode = CHOL_MODE_PLAIN or CHOL_MODE_MIXED, or -1 for both forms;
  * nbo = outer block width of the blocked path (a multiple of 128, <= 4096),
- * or -1 to keep the whole-matrix LDS kernel (only for n_hi <= 256).  The
+ * -1 to keep the whole-matrix LDS kernel (only for n_hi <= 256), or -2 for
+ * the fused one-workgroup-per-matrix kernel (only for n_lo > 64).  The
  * first matching row wins; unmatched shapes use built-in defaults; rows = 0
  * clears the table.  Replaces the reference's hand-written per-shape tables
  * (submission.py:2263-2281 _SPLIT32_SHAPES, :3262-3281 _LARGE_CFG,

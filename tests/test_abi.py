@@ -112,6 +112,14 @@ def test_tuning_table_validation():
     assert lib.chol_set_tuning(bad.ctypes.data, 1) == _lib.E_ARG
     bad = np.array([[300, 512, 1, 10, 0, 100]], dtype=np.int64)  # not a multiple of 128
     assert lib.chol_set_tuning(bad.ctypes.data, 1) == _lib.E_ARG
+    bad = np.array([[64, 512, 1, 10, 0, -2]], dtype=np.int64)  # fused route down to n = 64
+    assert lib.chol_set_tuning(bad.ctypes.data, 1) == _lib.E_ARG
+    fused = np.array([[65, 700, 100, 1000, -1, -2]], dtype=np.int64)
+    assert lib.chol_set_tuning(fused.ctypes.data, 1) == 0
+    assert lib.chol_route(640, 512, 0, ctypes.byref(nbo)) == 4 and nbo.value == 0
+    assert lib.chol_route(640, 100, 1, ctypes.byref(nbo)) == 4
+    assert lib.chol_route(99, 512, 1, ctypes.byref(nbo)) == 3
+    assert lib.chol_route(640, 32, 1, ctypes.byref(nbo)) == 1
     assert lib.chol_set_tuning(None, 0) == 0
     from gpumode_cholesky_amd import autotune
     autotune.load_table()

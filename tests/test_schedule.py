@@ -280,4 +280,4 @@ def test_route_and_counters_host_only():
     assert lib.chol_route(3, 300, 0, ctypes.byref(nbo)) == 3 and nbo.value % 128 == 0
     assert lib.chol_route(0, 300, 0, None) == 0
     out = (ctypes.c_int64 * 8)()
-    assert lib.chol_route_counters(out, 8) == 6
+    assert lib.chol_route_counters(out, 8) == 7
