@@ -121,6 +121,39 @@ static int side_stream(SideStream*& out) {
   return 0;
 }
 
+int side_begin(hipStream_t s, hipStream_t* s2, std::unique_lock<std::mutex>& lk) {
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (s && hipStreamIsCapturing(s, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone) {
+    *s2 = s;  // under capture: one stream (see chol_potrf_f32)
+    return 0;
+  }
+  SideStream* side = nullptr;
+  int rc = side_stream(side);
+  if (rc) return rc;
+  lk = std::unique_lock<std::mutex>(side->enqueue);
+  hipError_t e;
+  if ((e = hipEventRecord(side->to_side, s)) != hipSuccess ||
+      (e = hipStreamWaitEvent(side->s2, side->to_side, 0)) != hipSuccess) {
+    set_error("side stream: event ordering failed: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  *s2 = side->s2;
+  return 0;
+}
+
+int side_end(hipStream_t s, hipStream_t s2) {
+  if (s2 == s) return 0;
+  SideStream* side = nullptr;
+  int rc = side_stream(side);
+  if (rc) return rc;
+  hipError_t e;
+  if ((e = hipEventRecord(side->from_side, s2)) != hipSuccess || (e = hipStreamWaitEvent(s, side->from_side, 0)) != hipSuccess) {
+    set_error("side stream: event ordering failed: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  return 0;
+}
+
 // Compute units of the current device (hipDeviceAttributeMultiprocessorCount),
 // read once per device; the blocked driver sizes its idle-CU work by it.
 static int device_cus() {

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "chol_amd.h"
@@ -306,6 +307,14 @@ int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int 
 
 // Diagonal block width of the blocked path.
 constexpr int kNB = 128;
+
+// The library's side stream of the current device for drivers other than
+// the dense blocked path: side_begin makes it wait for s's work so far and
+// returns it in *s2 (s itself while s is being captured into a graph),
+// holding the device's enqueue lock in lk; side_end makes s wait for the
+// work enqueued on *s2 since.
+int side_begin(hipStream_t s, hipStream_t* s2, std::unique_lock<std::mutex>& lk);
+int side_end(hipStream_t s, hipStream_t s2);
 
 // Dense dispatch behind chol_potrf_f32 (argument checks, route, launches);
 // side_ok = false keeps all of the blocked route on stream s.  `who` names

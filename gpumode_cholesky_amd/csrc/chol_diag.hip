@@ -837,7 +837,8 @@ __device__ __forceinline__ void zero_upper_one(const Mat& M) {
 // CU and its L1, so after a barrier a wave's loads see the global stores
 // the others made before it.  p.first (dense, A and L with the same
 // leading dimension): step 0 reads the old values it first touches from A
-// (no copy of tril(A) into L); otherwise L already holds tril(A).
+// (no copy of tril(A) into L) and the zeros above the diagonal are stored
+// at the end; otherwise L already holds tril(A), zeros included.
 template <bool VEC, bool SPLIT>
 __device__ __forceinline__ void fused_matrix(const Batch& p, int64_t b, float* __restrict__ Wb, DiagLds& sm) {
   const Mat M = mat_at(p, b);
@@ -865,7 +866,7 @@ __device__ __forceinline__ void fused_matrix(const Batch& p, int64_t b, float* _
                                                n, lds, true);
       }
   }
-  zero_upper_one(M);
+  if (p.first) zero_upper_one(M);
 }
 
 // One workgroup per matrix (grid.x = the batch; workgroups are dispatched

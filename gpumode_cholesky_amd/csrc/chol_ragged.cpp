@@ -3,12 +3,17 @@
 //
 // The host sorts the matrices by order (largest first) and uploads one
 // descriptor table (pointers, orders, leading dimensions of L, original
-// positions) into the workspace.  The sorted list splits into two size
-// classes, each one launch sequence over its contiguous slice:
+// positions) into the workspace.  Every matrix above order 64 is copied
+// first (L = tril(A), k_copy_lower).  The sorted list then splits into
+// three size classes, each one launch sequence over its contiguous slice:
 //   n <= 64 : persistent wavefront kernel (k_potrf_small_queue), one
 //             launch per padded order 64/32/16, fed by an atomic work queue
 //             over the sorted slice;
-//   n > 64  : L = tril(A) (k_copy_lower), then the blocked path in lock
+//   64 < n <= 512 (or as the tuning table routes orders with nbo = -2):
+//             one launch of the fused kernel (k_potrf_fused), a workgroup
+//             per matrix that runs its whole factorisation, the largest
+//             matrices dispatched first;
+//   larger  : the blocked path in lock
 //             step: outer blocks of nbo columns (the tuning table's width
 //             for the largest order, else 256, 512 from 1024); inside one,
 //             128-column steps whose diagonal-block launch touches the
@@ -21,6 +26,8 @@
 //             past a smaller matrix exit).  Rows of L padded to a multiple
 //             of 4 (ld_host) give every kernel after the copy float4
 //             access whatever the orders.
+// The first two classes run on the library's side stream, beside the lock
+// step on the caller's stream.
 // The reference factors one fixed (batch, n) per call (custom_kernel,
 // submission.py:4101-4268); this generalises its per-shape dispatch.
 #include <algorithm>
@@ -33,6 +40,9 @@ namespace chol {
 namespace {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Largest order that goes to the fused class when no tuning row matches.
+constexpr int64_t kFusedMax = 512;
 
 struct Layout {
   size_t queues, Ap, Lp, ns, orig, lds, winv, total;
@@ -155,13 +165,16 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
     return i;
   };
   const int64_t c64 = first_at_most(64), c32 = first_at_most(32), c16 = first_at_most(16), cz = first_at_most(0);
-
-  // n > 64: lock-step blocked path over the prefix [0, c64), in chunks of at
-  // most 65535 matrices (the kernels put the matrix on grid.y).
-  const bool split = mode == CHOL_MODE_MIXED;
-  for (int64_t c0 = 0; c0 < c64; c0 += 65535) {
-    const int64_t cnt = std::min<int64_t>(c64 - c0, 65535);
-    const int32_t* cns = ns + c0;  // host copy of the chunk's orders (sorted, largest first)
+  // The fused class [cF, c64): the smallest orders above 64 that the
+  // tuning table routes to the fused kernel for a batch of `count`
+  // (nbo = -2), or, with no matching row, those up to kFusedMax.
+  auto fused_order = [&](int64_t n) {
+    const int64_t t = tuned_outer_width(n, count, mode);
+    return t == 0 ? n <= kFusedMax : t == -2;
+  };
+  int64_t cF = c64;
+  while (cF > 0 && fused_order(ns[cF - 1])) --cF;
+  auto slice = [&](int64_t c0, int64_t cnt) {
     Batch q = base;
     q.Ap = base.Ap + c0;
     q.Lp = base.Lp + c0;
@@ -169,9 +182,44 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
     q.orig = base.orig + c0;
     q.lds = base.lds + c0;
     q.batch = cnt;
-    q.n = cns[0];
-    float* winv = reinterpret_cast<float*>(w + lay.winv) + c0 * kNB * kNB;
-    if ((rc = launch_copy_lower(q, s))) return rc;
+    q.n = ns[c0];
+    return q;
+  };
+  float* const winv0 = reinterpret_cast<float*>(w + lay.winv);
+
+  // L = tril(A) for every matrix above 64, in chunks of at most 65535 (the
+  // copy puts the matrix on grid.y).
+  for (int64_t c0 = 0; c0 < c64; c0 += 65535)
+    if ((rc = launch_copy_lower(slice(c0, std::min<int64_t>(c64 - c0, 65535)), s))) return rc;
+
+  // Mid orders [cF, c64): one fused launch, a workgroup per matrix,
+  // largest first; then the n <= 64 queues.  Both on the side stream when
+  // the lock-step class below has work for the caller's stream, so the
+  // classes overlap.
+  const bool split = mode == CHOL_MODE_MIXED;
+  hipStream_t s2 = s;
+  std::unique_lock<std::mutex> side_lock;
+  if (cF > 0 && cz > cF && (rc = side_begin(s, &s2, side_lock))) return rc;
+  if (c64 > cF && (rc = launch_fused(slice(cF, c64 - cF), winv0 + cF * kNB * kNB, split, s2))) return rc;
+  // n <= 64: persistent wavefronts (4 per workgroup, 64/np matrices each).
+  auto waves = [](int64_t cnt, int np) { return (cnt * np + 63) / 64; };
+  if ((rc = launch_small_queue(base, c64, c32 - c64, 64, queues + 2,
+                               (int)std::min<int64_t>((waves(c32 - c64, 64) + 3) / 4, 512), s2)))
+    return rc;
+  if ((rc = launch_small_queue(base, c32, c16 - c32, 32, queues + 3,
+                               (int)std::min<int64_t>((waves(c16 - c32, 32) + 3) / 4, 1024), s2)))
+    return rc;
+  if ((rc = launch_small_queue(base, c16, cz - c16, 16, queues + 4,
+                               (int)std::min<int64_t>((waves(cz - c16, 16) + 3) / 4, 1024), s2)))
+    return rc;
+
+  // The larger orders: lock-step blocked path over the prefix [0, cF), in chunks
+  // of at most 65535 matrices (the kernels put the matrix on grid.y).
+  for (int64_t c0 = 0; c0 < cF; c0 += 65535) {
+    const int64_t cnt = std::min<int64_t>(cF - c0, 65535);
+    const int32_t* cns = ns + c0;  // host copy of the chunk's orders (sorted, largest first)
+    Batch q = slice(c0, cnt);
+    float* winv = winv0 + c0 * kNB * kNB;
     auto active = [&](int64_t above) {  // matrices of the chunk with n > above
       int64_t a = 0;
       while (a < cnt && cns[a] > above) ++a;
@@ -196,17 +244,7 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
       if ((rc = launch_syrk(q, E, -1, K0, (int)nbo, split, s))) return rc;
     }
   }
-  // n <= 64: persistent wavefronts (4 per workgroup, 64/np matrices each).
-  auto waves = [](int64_t cnt, int np) { return (cnt * np + 63) / 64; };
-  if ((rc = launch_small_queue(base, c64, c32 - c64, 64, queues + 2,
-                               (int)std::min<int64_t>((waves(c32 - c64, 64) + 3) / 4, 512), s)))
-    return rc;
-  if ((rc = launch_small_queue(base, c32, c16 - c32, 32, queues + 3,
-                               (int)std::min<int64_t>((waves(c16 - c32, 32) + 3) / 4, 1024), s)))
-    return rc;
-  if ((rc = launch_small_queue(base, c16, cz - c16, 16, queues + 4,
-                               (int)std::min<int64_t>((waves(cz - c16, 16) + 3) / 4, 1024), s)))
-    return rc;
+  if ((rc = side_end(s, s2))) return rc;
   note_call(RC_RAGGED);
   return 0;
 }

@@ -3,7 +3,9 @@
 Each matrix of a mixed-order list must match the FP64 oracle to the same
 FP32 tolerance as the dense path (reconstruction <= 8 sqrt(n) eps32), be
 bit-identical to the dense path where both run the same kernel arithmetic
-(n <= 64), and report info per matrix in the caller's order.
+(n <= 64), and report info per matrix in the caller's order.  Orders
+65..512 run the fused kernel by default and the lock-step blocked path when
+the tuning table routes them elsewhere (LOCKSTEP); both are tested.
 """
 import math
 
@@ -11,7 +13,7 @@ import numpy as np
 import pytest
 import torch
 
-from gpumode_cholesky_amd import cholesky, harness
+from gpumode_cholesky_amd import autotune, cholesky, harness
 from gpumode_cholesky_amd.ragged import cholesky_ragged
 
 pytestmark = pytest.mark.gpu
@@ -19,11 +21,24 @@ DEV = torch.device("cuda", 0)
 EPS32 = float(np.finfo(np.float32).eps)
 
 SIZES = [1, 2, 16, 17, 31, 32, 33, 48, 64, 65, 100, 128, 129, 200, 256, 257, 300, 512, 700, 1024, 5, 64, 32]
+# Every order above 64 on the lock-step path (outer blocks of 256).
+LOCKSTEP = [[65, 32768, 1, 1 << 30, 256]]
 
 
-def test_ragged_mixed_orders_against_oracle(oracle_mod):
+def _with_table(table, fn):
+    try:
+        if table is not None:
+            autotune.set_table(table)
+        return fn()
+    finally:
+        autotune.load_table()
+
+
+@pytest.mark.parametrize("route", ["default", "lockstep"])
+def test_ragged_mixed_orders_against_oracle(oracle_mod, route):
     mats = [harness.generate_input(1, n, 2, 700 + i, device="cpu")[0] for i, n in enumerate(SIZES)]
-    outs, info = cholesky_ragged([m.to(DEV) for m in mats], return_info=True)
+    outs, info = _with_table(LOCKSTEP if route == "lockstep" else None,
+                             lambda: cholesky_ragged([m.to(DEV) for m in mats], return_info=True))
     assert (info.cpu() == 0).all()
     for i, (a, L) in enumerate(zip(mats, outs)):
         n = a.shape[0]
@@ -37,12 +52,15 @@ def test_ragged_mixed_orders_against_oracle(oracle_mod):
             assert torch.equal(dense, L), f"matrix {i} n={n}: differs from the dense path"
 
 
+@pytest.mark.parametrize("route", ["default", "lockstep"])
 @pytest.mark.parametrize("sizes", [[65, 100, 128, 77, 128], [129, 200, 256, 65, 100, 150, 255, 33]])
-def test_ragged_mid_orders_only(oracle_mod, sizes):
-    # Orders 65..256 run the lock-step blocked path; with no larger matrix,
-    # the block that ends the largest is every matrix's last (no inverse).
+def test_ragged_mid_orders_only(oracle_mod, sizes, route):
+    # Mid orders alone: the fused kernel (default; no lock-step class, so no
+    # side stream), or the lock-step path, where with no larger matrix the
+    # block that ends the largest is every matrix's last (no inverse).
     mats = [harness.generate_input(1, n, 3, 900 + i, device="cpu")[0] for i, n in enumerate(sizes)]
-    outs, info = cholesky_ragged([m.to(DEV) for m in mats], return_info=True)
+    outs, info = _with_table(LOCKSTEP if route == "lockstep" else None,
+                             lambda: cholesky_ragged([m.to(DEV) for m in mats], return_info=True))
     assert (info.cpu() == 0).all()
     for i, (a, L) in enumerate(zip(mats, outs)):
         n = a.shape[0]
@@ -96,15 +114,18 @@ def test_ragged_workload_gate(mode):
     assert worst <= (1e-5 if mode == "mixed" else 8 * math.sqrt(2048) * EPS32)
 
 
-def test_ragged_more_than_65535_large_matrices():
-    # 65600 matrices of orders 65..72 (all on the blocked lock-step path,
-    # whose kernels put the matrix on grid.y): the call splits into chunks of
-    # at most 65535; sampled factors on both sides of the split, and info.
+@pytest.mark.parametrize("route", ["default", "lockstep"])
+def test_ragged_more_than_65535_large_matrices(route):
+    # 65600 matrices of orders 65..72: on the lock-step path (whose kernels
+    # put the matrix on grid.y) the call splits into chunks of at most
+    # 65535; the fused kernel takes them in one launch (matrix on grid.x).
+    # Sampled factors on both sides of the split, and info.
     count = 65600
     ns = [65 + (i % 8) for i in range(count)]
     base = {n: harness.generate_input(1, n, 2, 880 + n, device=DEV)[0] for n in set(ns)}
     mats = [base[n] for n in ns]
-    outs, info = cholesky_ragged(mats, return_info=True)
+    outs, info = _with_table(LOCKSTEP if route == "lockstep" else None,
+                             lambda: cholesky_ragged(mats, return_info=True))
     torch.cuda.synchronize()
     assert (info == 0).all().item()
     for i in (0, 7, 65534, 65535, 65536, count - 1):
