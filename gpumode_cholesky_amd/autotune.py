@@ -23,6 +23,7 @@ from . import _lib
 
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 NBO_CANDIDATES = (128, 256, 512, 1024, 2048)
+FUSED = -2  # table value of the fused route
 
 
 def set_table(rows) -> None:
@@ -46,9 +47,12 @@ def load_table(path: str = DEFAULT_TABLE) -> list:
 
 
 def candidates(n: int):
+    """Table values to time for order n: -1 (whole-matrix LDS kernel,
+    n <= 256), -2 (fused one-workgroup-per-matrix kernel, n > 64) and the
+    blocked path's outer widths up to n."""
     out = [-1] if n <= 256 else []
     if n > 64:
-        out += [c for c in NBO_CANDIDATES if c <= max(64, n)]
+        out += [FUSED] + [c for c in NBO_CANDIDATES if c <= max(64, n)]
     return out
 
 

@@ -56,18 +56,29 @@ struct TuneRow {
 static std::mutex g_tune_mu;
 static std::vector<TuneRow> g_tune;
 
-// Default when no row matches: the blocked path with one outer block up to
-// n = 256 (measured 1.2-2.2x faster than the whole-matrix LDS kernel from
-// 16x100 to 4096x128 and 512x256), the LDS kernel only for batches of 8192
-// and more (8192x96: 663 us against 721 us), then outer blocks of 256
-// columns (512 from n = 12288, where the trailing matrix outgrows the
-// 256 MiB Infinity Cache and the SYRK's C traffic dominates).
+// Default when no row matches: the fused kernel for 128 < n <= 512 when
+// the batch fills a 256-CU part one to a few times over (against the
+// blocked path, dense inputs, both forms, one MI355X: 256..512 x 160..256
+// 1.02-1.16x; 256..4096 x 300 1.16-1.55x; 256..2048 x 384 1.03-1.29x;
+// 256..1024 x 448 1.04-1.29x; 256..512 x 512 0.98-1.12x; fewer matrices
+// leave CUs idle while each runs its chain, more favour the blocked path's
+// wide launches: 4096 x 256 0.86-0.88x); else the blocked path with one
+// outer block up to n = 256 (measured 1.2-2.2x faster than the
+// whole-matrix LDS kernel from 16x100 to 4096x128 and 512x256), the LDS
+// kernel only for batches of 8192 and more (8192x96: 663 us against
+// 721 us), then outer blocks of 256 columns (512 from n = 12288, where the
+// trailing matrix outgrows the 256 MiB Infinity Cache and the SYRK's C
+// traffic dominates).
 static int64_t pick_nbo(int64_t n, int64_t batch, int32_t mode) {
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (const TuneRow& r : g_tune)
       if (n >= r.n_lo && n <= r.n_hi && batch >= r.b_lo && batch <= r.b_hi && (r.mode < 0 || r.mode == mode))
         return r.nbo;
+  }
+  if (n > kNB && n <= 512 && batch >= 256) {
+    const int64_t cap = n <= 256 ? 512 : n <= 384 ? 4096 : n <= 448 ? 1024 : 512;
+    if (batch <= cap) return kRouteFused;
   }
   if (n <= kLdsMaxN) return batch >= 8192 ? -1 : (n <= kNB ? kNB : 2 * kNB);
   return n >= 12288 ? 512 : 256;

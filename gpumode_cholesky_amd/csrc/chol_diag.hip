@@ -421,8 +421,10 @@ __device__ __forceinline__ void pre_tiles(float* Ls, const __bf16* Ph, int li, i
 // block: the block's update is applied to it in LDS first (bfirst: that
 // panel is the first, the block's old values come from A).  from_a_rt: the
 // block is staged from A, as for FROM_A, chosen at run time.  opaque: the
-// thread index is hidden from the optimiser (see gemm_tile_once), for
-// callers that run this in a loop with other large code.
+// thread index is hidden from the optimiser, for a caller that runs this
+// in a loop with other large code (the fused kernel): otherwise addresses
+// derived from it are hoisted out of the loop, stay live across the other
+// phases and spill.
 template <bool FROM_A, bool LAST, bool PRE>
 __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, float* __restrict__ Wb, DiagLds& sm,
                                            int bk0, int bfirst, bool from_a_rt = false, bool opaque = false) {
@@ -826,11 +828,175 @@ __device__ __forceinline__ void zero_upper_one(const Mat& M) {
   }
 }
 
+// A stream of one-shot (K = 128) tiles that share one operand, for the
+// fused kernel: C[i0 .. i0+64, j0 .. j0+128) for the 64-row tiles
+// i0 = ibeg, ibeg + 64, ... < n.  B (128 rows from brow0, the 128 columns
+// from kb) is staged into LDS once; each tile's A rows (the 128 columns
+// from ka) and old C values are loaded into registers while the previous
+// tile's MFMAs run, so a tile costs its MFMA time plus one LDS staging,
+// not a global-memory round trip.
+//   MODE 0: C = Cold - A B^T on {j <= i, i < n, j < jlim} (Cold from Cs);
+//   MODE 1: C = A B^T for i < n, j < jlim (the panel: C may be A's own
+//           rows; a tile's rows are in LDS before any of its stores, and
+//           the prefetched rows are the next tile's).
+// SPLIT: BF16x3 products (x = hi + lo; hi*hi + hi*lo + lo*hi), FP32
+// accumulation.  tid: the thread index (opaque to the optimiser).
+template <int MODE, bool VEC, bool SPLIT>
+__device__ __forceinline__ void stream_tiles(const float* A, int64_t lda, int ka, int ibeg, int n, const float* B,
+                                             int64_t ldb, int brow0, int blim, int kb, float* C, const float* Cs,
+                                             int64_t ldc, int j0, int jlim, float* __restrict__ lds, int tid) {
+  constexpr int TM = 64, TN = 128;
+  constexpr int WM = 32, WN = 64, FM = WM / 16, FN = WN / 16;
+  constexpr int PA = TM * K1 / 4 / kThreads;
+  constexpr int PB = TN * K1 / 4 / kThreads;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 1) * WM;
+  const int wn = (wave & 1) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+  __bf16* const Bh = reinterpret_cast<__bf16*>(lds);
+  __bf16* const Bl = Bh + TN * K1H;
+  __bf16* const Ah = Bl + TN * K1H;
+  __bf16* const Al = Ah + TM * K1H;
+  float* const Bs = lds;
+  float* const As = lds + TN * K1P;
+
+  float4 ra[PA];
+  float oldn[FM][FN][4];
+  auto load_tile = [&](int i0) {
+    load_rows<TM, VEC, K1>(ra, A, lda, i0, n, ka, tid);
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = i0 + wm + a * 16 + 4 * lg + v;
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            const int j = j0 + wn + b * 16 + li;
+            const bool ok = i < n && j < jlim && j <= i;
+            oldn[a][b][v] = gld(Cs + (ok ? (int64_t)i * ldc + j : 0));  // masked: C[0][0], always valid
+          }
+        }
+    }
+  };
+  __syncthreads();  // the LDS is free (the caller's previous stream or block)
+  {
+    float4 rb[PB];
+    load_rows<TN, VEC, K1>(rb, B, ldb, brow0, blim, kb, tid);
+    if (ibeg < n) load_tile(ibeg);
+    if constexpr (SPLIT)
+      store_rows_split<TN, K1, K1H>(Bh, Bl, rb, tid);
+    else
+      store_rows<TN, K1, K1P>(Bs, rb, tid);
+  }
+  for (int i0 = ibeg; i0 < n; i0 += TM) {
+    __syncthreads();  // the previous tile's LDS reads are done
+    if constexpr (SPLIT)
+      store_rows_split<TM, K1, K1H>(Ah, Al, ra, tid);
+    else
+      store_rows<TM, K1, K1P>(As, ra, tid);
+    float oldc[FM][FN][4];
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) oldc[a][b][v] = oldn[a][b][v];
+    }
+    __syncthreads();
+    if (i0 + TM < n) load_tile(i0 + TM);  // in flight during the MFMAs below
+
+    floatx4 acc[FM][FN];
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (SPLIT) {
+#pragma unroll
+      for (int ks = 0; ks < K1; ks += 32) {
+        bf16x8 fah[FM], fal[FM], fbh[FN], fbl[FN];
+#pragma unroll
+        for (int a = 0; a < FM; ++a) {
+          const int o = (wm + a * 16 + li) * K1H + ks + 8 * lg;
+          fah[a] = *reinterpret_cast<const bf16x8*>(&Ah[o]);
+          fal[a] = *reinterpret_cast<const bf16x8*>(&Al[o]);
+        }
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          const int o = (wn + b * 16 + li) * K1H + ks + 8 * lg;
+          fbh[b] = *reinterpret_cast<const bf16x8*>(&Bh[o]);
+          fbl[b] = *reinterpret_cast<const bf16x8*>(&Bl[o]);
+        }
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fal[a], fbh[b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbl[b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fah[a], fbh[b], acc[a][b], 0, 0, 0);
+          }
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < K1; ks += 16) {
+        float4 fa[FM], fb[FN];
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+          fa[a] = *reinterpret_cast<const float4*>(As + (wm + a * 16 + li) * K1P + ks + 4 * lg);
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+          fb[b] = *reinterpret_cast<const float4*>(Bs + (wn + b * 16 + li) * K1P + ks + 4 * lg);
+        // one k at a time over all FM x FN accumulators: consecutive MFMAs
+        // are independent
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].x, fb[b].x, acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].y, fb[b].y, acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].z, fb[b].z, acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a].w, fb[b].w, acc[a][b], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wm + a * 16 + 4 * lg + v;
+        if (i >= n) continue;
+        float* crow = C + (int64_t)i * ldc;
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          const int j = j0 + wn + b * 16 + li;
+          if constexpr (MODE == 0) {
+            if (j < jlim && j <= i) gst(crow + j, oldc[a][b][v] - acc[a][b][v]);
+          } else {
+            if (j < jlim) gst(crow + j, acc[a][b][v]);
+          }
+        }
+      }
+  }
+}
+static_assert((2 * 128 * K1H + 2 * 64 * K1H) * 2 <= (int)sizeof(DiagLds) &&
+                  (128 + 64) * K1P * 4 <= (int)sizeof(DiagLds),
+              "stream_tiles staging exceeds the fused kernel's LDS");
+
 // The whole factorisation of matrix b by one workgroup: for each 128
-// columns, the diagonal block and its inverse W (diag_block, W at Wb), the
-// panel L21 = A21 W^T in 64-row one-shot tiles, and the right-looking
-// update of the trailing lower triangle, C -= L21 L21^T, in 64 x 128
-// one-shot tiles (BF16x3 products when SPLIT); then the zeros above the
+// columns, the diagonal block and its inverse (diag_block, W at Wb), the
+// panel L21 = A21 W^T and the right-looking update of the trailing lower
+// triangle, C -= L21 L21^T, as streams of 64-row tiles (stream_tiles; the
+// update one stream per 128-column tile of C); then the zeros above the
 // diagonal.  The matrix lives in L, where the caches keep the part the
 // workgroup works on; every tile is staged through LDS.  Phases are
 // separated by workgroup barriers only: all waves of a workgroup share one
@@ -852,19 +1018,15 @@ __device__ __forceinline__ void fused_matrix(const Batch& p, int64_t b, float* _
     // unused, and its upper right quadrant is left to zero_upper_one.
     diag_block<false, false, false>(p, b, k0, Wb, sm, 0, 0, first, true);
     if (k0 + NB >= n) break;
-    const float* As = first ? M.A : M.L;
-    for (int i0 = k0 + NB; i0 < n; i0 += 64) {
-      __syncthreads();  // W and L11 stored; the LDS free
-      gemm_tile_once<64, NB, 1, VEC, SPLIT>(As, M.ld, i0, n, k0, Wb, NB, 0, NB, 0, M.L, M.L, M.ld, i0, k0, n,
-                                            k0 + NB, lds, true);
-    }
+    // The thread index hidden from the optimiser (as in diag_block): no
+    // tile addresses hoisted out of this loop across the block code.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    stream_tiles<1, VEC, SPLIT>(first ? M.A : M.L, M.ld, k0, k0 + NB, n, Wb, NB, 0, NB, 0, M.L, M.L, M.ld, k0,
+                                k0 + NB, lds, tid);
     const float* Cs = first ? M.A : M.L;
     for (int j0 = k0 + NB; j0 < n; j0 += 128)
-      for (int i0 = j0; i0 < n; i0 += 64) {
-        __syncthreads();  // the panel stored; the LDS free
-        gemm_tile_once<64, 128, 0, VEC, SPLIT>(M.L, M.ld, i0, n, k0, M.L, M.ld, j0, n, k0, M.L, Cs, M.ld, i0, j0, n,
-                                               n, lds, true);
-      }
+      stream_tiles<0, VEC, SPLIT>(M.L, M.ld, k0, j0, n, M.L, M.ld, j0, n, k0, M.L, Cs, M.ld, j0, n, lds, tid);
   }
   if (p.first) zero_upper_one(M);
 }

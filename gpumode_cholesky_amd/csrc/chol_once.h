@@ -110,19 +110,13 @@ template <int BM, int BN, int MODE, bool VEC, bool SPLIT>
 __device__ __forceinline__ void gemm_tile_once(const float* Abase, int64_t lda, int a_row0, int a_lim, int ka,
                                                const float* Bbase, int64_t ldb, int b_row0, int b_lim, int kb,
                                                float* Cbase, const float* Csrc, int64_t ldc, int c_row0,
-                                               int c_col0, int c_ilim, int c_jlim, float* __restrict__ lds,
-                                               bool opaque = false) {
+                                               int c_col0, int c_ilim, int c_jlim, float* __restrict__ lds) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int PA = BM * K1 / 4 / kThreads;
   constexpr int PB = BN * K1 / 4 / kThreads;
 
-  int tid = threadIdx.x;
-  // opaque: called in a loop next to other large inlined code (the fused
-  // kernel); hiding the thread index keeps the compiler from hoisting this
-  // tile's lane-dependent addresses out of the loop, where they would stay
-  // live across the other phases and spill.
-  if (opaque) asm volatile("" : "+v"(tid));
+  const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = (wave >> 1) * WM;

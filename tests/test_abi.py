@@ -121,6 +121,12 @@ def test_tuning_table_validation():
     assert lib.chol_route(99, 512, 1, ctypes.byref(nbo)) == 3
     assert lib.chol_route(640, 32, 1, ctypes.byref(nbo)) == 1
     assert lib.chol_set_tuning(None, 0) == 0
+    # built-in defaults: the fused kernel where a batch of mid orders fills
+    # the GPU one to a few times over, the blocked path outside
+    assert lib.chol_route(256, 300, 0, None) == 4 and lib.chol_route(4096, 300, 1, None) == 4
+    assert lib.chol_route(512, 512, 1, None) == 4 and lib.chol_route(640, 512, 0, None) == 3
+    assert lib.chol_route(255, 300, 0, None) == 3 and lib.chol_route(512, 128, 0, None) == 3
+    assert lib.chol_route(1024, 200, 0, None) == 3 and lib.chol_route(8192, 200, 0, None) == 2
     from gpumode_cholesky_amd import autotune
     autotune.load_table()
 
