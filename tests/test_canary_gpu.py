@@ -3,8 +3,10 @@ and the input A sit inside larger allocations whose margins hold a
 sentinel pattern; after the call (our kernels only, nothing else on the
 GPU) the margins must be unchanged and A untouched.  Covers every route
 (register n <= 64, LDS, blocked with partial last blocks: 300 = 2*128+44,
-320 = 2*128+64, 1100 = 8*128+76), both forms, unaligned leading dimensions,
-the ragged entry point, the triangular inverse and the backward."""
+320 = 2*128+64, 1100 = 8*128+76, fused one-workgroup-per-matrix), both
+forms, unaligned leading dimensions, the ragged entry point (its orders
+65..512 on the fused kernel, the larger ones in lock step), the triangular
+inverse and the backward."""
 import numpy as np
 import pytest
 import torch
@@ -90,6 +92,20 @@ def test_no_store_outside_buffers_lds_route(n):
         a = harness.generate_input(3, n, 2, 8900 + n, device=DEV)
         _potrf_guarded(a, n, 0)
         _potrf_guarded(a, n + 1, 0)
+    finally:
+        autotune.load_table()
+
+
+@pytest.mark.parametrize("n", [65, 100, 128, 129, 200, 256, 300, 320, 383, 512, 700])
+def test_no_store_outside_buffers_fused_route(n):
+    try:
+        autotune.set_table([[n, n, 1, 1 << 20, -2]])
+        a = harness.generate_input(3, n, 2, 9200 + n, device=DEV)
+        for mode in (0, 1):
+            for ld in (n, n + 3):
+                L = _potrf_guarded(a, ld, mode)
+                r = harness.fp64_residual(a, L.contiguous())
+                assert r < 1e-5, (n, ld, mode, r)
     finally:
         autotune.load_table()
 
