@@ -5,9 +5,9 @@
 // the caller's stream (no side stream or events are created).
 //
 // The array (`scratch`, 64 MiB per device, held until the process ends)
-// carries the blocked route's per-matrix 128 x 128 block inverses, 64 KiB a
-// matrix, so one call takes at most 1024 matrices of order > 64; orders
-// <= 64 and the whole-matrix LDS route use none of it.  Every call on a
+// carries the blocked and fused routes' per-matrix 128 x 128 block
+// inverses, 64 KiB a matrix, so one call takes at most 1024 matrices of
+// order > 64; orders <= 64 and the whole-matrix LDS route use none of it.  Every call on a
 // device shares that device's array: calls that overlap on one device (two
 // streams, two host threads) race on it and must be serialised by the
 // caller.  Nothing needs to start at zero: each diagonal-block launch
