@@ -243,6 +243,25 @@ def extras(dev):
     torch.cuda.empty_cache()
     print("# extras: ragged done", file=sys.stderr, flush=True)
 
+    # The fused route (one workgroup per matrix) on a shape it takes by
+    # default, against the same shape forced onto the blocked path.
+    from gpumode_cholesky_amd import _lib, autotune
+    fu = {}
+    for b, n in ((2000, 300),):
+        a = harness.dense_input(b, n, 1.0, 2300, device=dev)
+        row = {"route": int(_lib.load().chol_route(b, n, 0, None))}
+        for mode in ("plain", "mixed"):
+            row[f"fused_us_{mode}"] = round(_event_us(lambda: cholesky(a, mode=mode), 5), 1)
+            try:
+                autotune.set_table([[n, n, b, b, 256]])
+                row[f"blocked_us_{mode}"] = round(_event_us(lambda: cholesky(a, mode=mode), 5), 1)
+            finally:
+                autotune.load_table()
+        fu[f"{b}x{n}"] = row
+        del a
+    out["fused_route"] = fu
+    print("# extras: fused route done", file=sys.stderr, flush=True)
+
     # Graph replay vs eager calls: the two smallest fixed shapes, and calls
     # that leave most CUs idle.  Each shape's rotated inputs (as many as the
     # timed protocol uses, min(50, 256 MiB / input)) in one graph of
