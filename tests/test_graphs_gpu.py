@@ -22,14 +22,17 @@ DEV = torch.device("cuda", 0)
                                           (2, 2048, "mixed"), (1, 4096, "mixed"), (1, 4096, "plain"),
                                           (4, 256, "mixed"), (512, 300, "mixed"), (512, 300, "plain")])
 def test_graph_replays_match_eager(batch, n, mode, branches):
-    # (512, 300): the fused route (its default for this shape), on dense
-    # inputs (the spectral generator's batched QR is slow at this size;
-    # replay equality does not depend on the input family).
+    # (512, 300): the fused route (its default for this shape).  Batches of
+    # 256 and more take dense inputs: the spectral generator's batched QR
+    # is slow there, and replay equality does not depend on the family.
     print(f"[graph] {batch}x{n} {mode} branches={branches}", flush=True)
-    if (batch, n) == (512, 300):
-        inputs = [harness.dense_input(batch, n, 1.0, 900 + i, device=DEV) for i in range(2)]
-    else:
-        inputs = [harness.config_input(batch, n, 900 + i, device=DEV) for i in range(2)]
+
+    def gen(seed):
+        if batch >= 256:
+            return harness.dense_input(batch, n, 1.0, seed, device=DEV)
+        return harness.config_input(batch, n, seed, device=DEV)
+
+    inputs = [gen(900 + i) for i in range(2)]
     eager = [cholesky(a, mode=mode) for a in inputs]
     g = GraphedCholesky(inputs, mode=mode, branches=branches)
     for rep in range(3):
@@ -38,11 +41,11 @@ def test_graph_replays_match_eager(batch, n, mode, branches):
         for i, out in enumerate(outs):
             assert torch.equal(out, eager[i]), f"replay {rep}, input {i} differs"
         # eager work between replays
-        other = cholesky(harness.config_input(batch, n, 5, device=DEV), mode=mode)
+        other = cholesky(gen(5), mode=mode)
         torch.cuda.synchronize()
         del other
     # refresh an input in place: the replay factors the new contents
-    fresh = harness.config_input(batch, n, 777, device=DEV)
+    fresh = gen(777)
     inputs[0].copy_(fresh)
     outs = g.replay()
     torch.cuda.synchronize()
