@@ -24,6 +24,8 @@ from . import _lib
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 NBO_CANDIDATES = (128, 256, 512, 1024, 2048)
 FUSED = -2  # table value of the fused route
+FUSED_MAX_N = 1024  # largest order the tuner times on the fused route
+kNB_MIN = 128  # the smallest outer width (one diagonal block)
 
 
 def set_table(rows) -> None:
@@ -48,11 +50,14 @@ def load_table(path: str = DEFAULT_TABLE) -> list:
 
 def candidates(n: int):
     """Table values to time for order n: -1 (whole-matrix LDS kernel,
-    n <= 256), -2 (fused one-workgroup-per-matrix kernel, n > 64) and the
-    blocked path's outer widths up to n."""
+    n <= 256), -2 (fused one-workgroup-per-matrix kernel, 64 < n <= 1024:
+    above that one matrix on one workgroup takes milliseconds to seconds)
+    and the blocked path's outer widths up to n."""
     out = [-1] if n <= 256 else []
+    if 64 < n <= FUSED_MAX_N:
+        out.append(FUSED)
     if n > 64:
-        out += [FUSED] + [c for c in NBO_CANDIDATES if c <= max(64, n)]
+        out += [c for c in NBO_CANDIDATES if c <= max(kNB_MIN, n)]
     return out
 
 
