@@ -15,8 +15,10 @@
 // The kernels that first touch a region of L read its old values from A, so
 // a dense call moves no copy of tril(A) into L; the first diagonal-block
 // launch zeros L's upper triangle with extra workgroups on the CUs that the
-// batch leaves idle, or k_zero_upper does when it leaves none.  (The ragged
-// path still copies first.)
+// batch leaves idle, or k_zero_upper does when it leaves none.  A ragged
+// call copies only each matrix's first 128 columns (k_copy_head), reads the
+// rest of A on first touch at A's own row stride, and zeros the rest of
+// the upper triangle by rows (k_zero_upper_rows).
 //
 // Reference counterparts: the 128-wide diag+inverse kernel e62_diag128
 // (submission.py:4694-4997), the panel-times-inverse bmm and trailing
@@ -95,6 +97,44 @@ __global__ __launch_bounds__(256) void k_copy_lower(Batch p) {
   }
 }
 
+// Ragged first touch, head part: L[i][j] = (j <= i ? A[i][j] : 0) for the
+// first min(n, 128) columns of every row (the first column block, which the
+// first diagonal block and panel read, so that they read L's aligned rows;
+// A's rows, at their own stride n, are 16-byte aligned only for n % 4 ==
+// 0).  Half a workgroup per row, workgroup x taking rows 2x, 2x + 1,
+// 2x + 2 gridDim.x, ...
+__global__ __launch_bounds__(256) void k_copy_head(Batch p) {
+  const Mat M = mat_at(p, blockIdx.y);
+  const int n = M.n;
+  const int w = n < kNB ? n : kNB;
+  const int t = threadIdx.x & (kNB - 1);
+  for (int i = 2 * blockIdx.x + (threadIdx.x >> 7); i < n; i += 2 * gridDim.x)
+    if (t < w) gst(M.L + (int64_t)i * M.ld + t, t <= i ? gld_nt(M.A + (int64_t)i * M.lda + t) : 0.f);
+}
+
+// Ragged first touch, the rest of the upper triangle: L[i][j] = 0 for
+// max(128, the chunk after row i's diagonal chunk) <= j < n, by rows
+// (workgroup x: rows x, x + gridDim.x, ...), float4 stores where L's row
+// starts 16-byte aligned.  The head columns are k_copy_head's, the
+// diagonal chunks the diagonal-block kernel's.
+__global__ __launch_bounds__(256) void k_zero_upper_rows(Batch p) {
+  const Mat M = mat_at(p, blockIdx.y);
+  const int n = M.n;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    float* row = M.L + (int64_t)i * M.ld;
+    const int j0 = max(kNB, ((i >> 2) + 1) * 4);
+    if (j0 >= n) continue;
+    if ((reinterpret_cast<uintptr_t>(row) & 15) == 0) {
+      const int full = (n - j0) >> 2;
+      for (int c = threadIdx.x; c < full; c += blockDim.x) gst4_nt(row + j0 + 4 * c, make_float4(0.f, 0.f, 0.f, 0.f));
+      const int jt = j0 + 4 * full + threadIdx.x;
+      if (jt < n) gst_nt(row + jt, 0.f);
+    } else {
+      for (int j = j0 + threadIdx.x; j < n; j += blockDim.x) gst_nt(row + j, 0.f);
+    }
+  }
+}
+
 // Zeros right of each row's diagonal chunk: L[i][c..c+3] = 0 for every
 // 4-column chunk c > i (the diagonal-block kernel stores each row's
 // diagonal chunk whole).  Used where the batch fills the GPU, so that the
@@ -148,6 +188,20 @@ static unsigned row_groups(int64_t n) {
 int launch_copy_lower(const Batch& p, hipStream_t s) {
   hipLaunchKernelGGL(k_copy_lower, dim3(row_groups(p.n), (unsigned)p.batch), dim3(256), 0, s, p);
   return launch_status("k_copy_lower");
+}
+
+int launch_copy_head(const Batch& p, hipStream_t s) {
+  if (p.batch <= 0) return 0;
+  const unsigned gx = (unsigned)((p.n + 15) / 16);  // 16 rows of the largest matrix per workgroup
+  hipLaunchKernelGGL(k_copy_head, dim3(gx, (unsigned)p.batch), dim3(256), 0, s, p);
+  return launch_status("k_copy_head");
+}
+
+int launch_zero_upper_rows(const Batch& p, hipStream_t s) {
+  if (p.batch <= 0 || p.n <= kNB) return 0;
+  const unsigned gx = (unsigned)((p.n + 7) / 8);  // 8 rows of the largest matrix per workgroup
+  hipLaunchKernelGGL(k_zero_upper_rows, dim3(gx, (unsigned)p.batch), dim3(256), 0, s, p);
+  return launch_status("k_zero_upper_rows");
 }
 
 int launch_zero_upper(const Batch& p, hipStream_t s) {

@@ -41,11 +41,15 @@ struct Batch {
   const int32_t* ns = nullptr;
   const int32_t* orig = nullptr;
   const int32_t* lds = nullptr;
-  // Dense blocked path, first touch of a region: the kernel reads the old
-  // values of its output region (C of a trailing update, A21 of a panel,
-  // the diagonal block) from A instead of L, so no copy of tril(A) into L
+  // First touch of a region: the kernel reads the old values of its output
+  // region (C of a trailing update, A21 of a panel, the diagonal block)
+  // from A (at A's row stride) instead of L, so no copy of tril(A) into L
   // precedes the factorisation.
   bool first = false;
+  // Fused kernel of a ragged call: the first 128 columns of L already hold
+  // tril(A) (k_copy_head); step 0's trailing update reads the old values of
+  // the columns beyond them from A, and the kernel zeros the upper triangle.
+  bool trailing_from_a = false;
 };
 
 // One matrix of a Batch, dense or ragged.
@@ -262,6 +266,8 @@ inline int launch_status(const char* what) {
 int launch_small(const Batch& p, hipStream_t s);                  // n <= 64
 int launch_lds(const Batch& p, hipStream_t s);                    // n <= 256
 int launch_copy_lower(const Batch& p, hipStream_t s);             // L = tril(A)
+int launch_copy_head(const Batch& p, hipStream_t s);              // the first 128 columns of L = tril(A)
+int launch_zero_upper_rows(const Batch& p, hipStream_t s);        // L = 0 above the diagonal beyond them
 int launch_zero_upper(const Batch& p, hipStream_t s);             // L = 0 right of the diagonal chunks
 bool vec_ok(const Batch& p);  // float4 global access allowed for every matrix of p
 // One-shot (K = kNB) SYRK tiles that a diagonal-block launch runs on the

@@ -441,11 +441,13 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
   const int nb = min(NB, M.n - k0);
   if (nb <= 0) return;  // (ragged) this matrix has no block at k0
   float* Lb = M.L + (int64_t)k0 * M.ld + k0;
-  // FROM_A: the first block, staged from A (nothing has written L yet).
+  // FROM_A: the first block, staged from A (nothing has written L yet),
+  // with A's own row stride (a ragged batch pads L's rows, not A's).
   const bool from_a = FROM_A || (PRE && bfirst != 0) || from_a_rt;
-  const float* Sb = from_a ? M.A + (int64_t)k0 * M.ld + k0 : Lb;
+  const int64_t lds_ = from_a ? M.lda : M.ld;  // row stride of the staging source
+  const float* Sb = from_a ? M.A + (int64_t)k0 * lds_ + k0 : Lb;
   const bool vec = ((M.ld & 3) == 0) && ((reinterpret_cast<uintptr_t>(M.L) & 15) == 0) &&
-                   (!from_a || (reinterpret_cast<uintptr_t>(M.A) & 15) == 0);
+                   (!from_a || ((lds_ & 3) == 0 && (reinterpret_cast<uintptr_t>(M.A) & 15) == 0));
   // ---- 1. stage: thread covers rows tid/32 + 8u, columns 4*(tid%32).
   //         All 16 loads are issued before the first LDS store, so the
   //         block costs one memory latency, not one per few rows. ----
@@ -457,7 +459,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
 #pragma unroll
     for (int u = 0; u < NB / 8; ++u) {
       const int r = (tid >> 5) + 8 * u;
-      stg[u] = gld4_nt(Sb + (int64_t)r * M.ld + (c4 <= r ? c4 : 0));
+      stg[u] = gld4_nt(Sb + (int64_t)r * lds_ + (c4 <= r ? c4 : 0));
     }
 #pragma unroll
     for (int u = 0; u < NB / 8; ++u) {
@@ -476,7 +478,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (r < nb) {
       if (c4 <= r) {
-        const float* src = Sb + (int64_t)r * M.ld + c4;
+        const float* src = Sb + (int64_t)r * lds_ + c4;
         if (vec && c4 + 3 < nb) {
           v = gld4(src);
         } else {
@@ -835,7 +837,8 @@ __device__ __forceinline__ void zero_upper_one(const Mat& M) {
 // from ka) and old C values are loaded into registers while the previous
 // tile's MFMAs run, so a tile costs its MFMA time plus one LDS staging,
 // not a global-memory round trip.
-//   MODE 0: C = Cold - A B^T on {j <= i, i < n, j < jlim} (Cold from Cs);
+//   MODE 0: C = Cold - A B^T on {j <= i, i < n, j < jlim} (Cold from Cs,
+//           row stride ldcs);
 //   MODE 1: C = A B^T for i < n, j < jlim (the panel: C may be A's own
 //           rows; a tile's rows are in LDS before any of its stores, and
 //           the prefetched rows are the next tile's).
@@ -844,7 +847,8 @@ __device__ __forceinline__ void zero_upper_one(const Mat& M) {
 template <int MODE, bool VEC, bool SPLIT>
 __device__ __forceinline__ void stream_tiles(const float* A, int64_t lda, int ka, int ibeg, int n, const float* B,
                                              int64_t ldb, int brow0, int blim, int kb, float* C, const float* Cs,
-                                             int64_t ldc, int j0, int jlim, float* __restrict__ lds, int tid) {
+                                             int64_t ldc, int64_t ldcs, int j0, int jlim, float* __restrict__ lds,
+                                             int tid) {
   constexpr int TM = 64, TN = 128;
   constexpr int WM = 32, WN = 64, FM = WM / 16, FN = WN / 16;
   constexpr int PA = TM * K1 / 4 / kThreads;
@@ -876,7 +880,7 @@ __device__ __forceinline__ void stream_tiles(const float* A, int64_t lda, int ka
           for (int b = 0; b < FN; ++b) {
             const int j = j0 + wn + b * 16 + li;
             const bool ok = i < n && j < jlim && j <= i;
-            oldn[a][b][v] = gld(Cs + (ok ? (int64_t)i * ldc + j : 0));  // masked: C[0][0], always valid
+            oldn[a][b][v] = gld(Cs + (ok ? (int64_t)i * ldcs + j : 0));  // masked: C[0][0], always valid
           }
         }
     }
@@ -1001,10 +1005,11 @@ static_assert((2 * 128 * K1H + 2 * 64 * K1H) * 2 <= (int)sizeof(DiagLds) &&
 // workgroup works on; every tile is staged through LDS.  Phases are
 // separated by workgroup barriers only: all waves of a workgroup share one
 // CU and its L1, so after a barrier a wave's loads see the global stores
-// the others made before it.  p.first (dense, A and L with the same
-// leading dimension): step 0 reads the old values it first touches from A
-// (no copy of tril(A) into L) and the zeros above the diagonal are stored
-// at the end; otherwise L already holds tril(A), zeros included.
+// the others made before it.  p.first: step 0 reads the old values it
+// first touches from A (at A's own row stride; no copy of tril(A) into L)
+// and the zeros above the diagonal are stored at the end;
+// p.trailing_from_a: the same, except that the first 128 columns are
+// already in L; otherwise L already holds tril(A), zeros included.
 template <bool VEC, bool SPLIT>
 __device__ __forceinline__ void fused_matrix(const Batch& p, int64_t b, float* __restrict__ Wb, DiagLds& sm) {
   const Mat M = mat_at(p, b);
@@ -1012,6 +1017,7 @@ __device__ __forceinline__ void fused_matrix(const Batch& p, int64_t b, float* _
   float* const lds = reinterpret_cast<float*>(&sm);
   for (int k0 = 0; k0 < n; k0 += NB) {
     const bool first = p.first && k0 == 0;
+    const bool cfirst = (p.first || p.trailing_from_a) && k0 == 0;  // trailing old values from A
     __syncthreads();  // the previous step's tiles are done with the LDS
     // One form of the block code for every step (a second inlined copy
     // makes the kernel spill): the last block also forms its inverse,
@@ -1022,13 +1028,18 @@ __device__ __forceinline__ void fused_matrix(const Batch& p, int64_t b, float* _
     // tile addresses hoisted out of this loop across the block code.
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
-    stream_tiles<1, VEC, SPLIT>(first ? M.A : M.L, M.ld, k0, k0 + NB, n, Wb, NB, 0, NB, 0, M.L, M.L, M.ld, k0,
-                                k0 + NB, lds, tid);
-    const float* Cs = first ? M.A : M.L;
+    // first touch: A21 and the old values of the trailing triangle from A,
+    // with A's own row stride
+    const float* As = first ? M.A : M.L;
+    const int64_t lda = first ? M.lda : M.ld;
+    stream_tiles<1, VEC, SPLIT>(As, lda, k0, k0 + NB, n, Wb, NB, 0, NB, 0, M.L, M.L, M.ld, M.ld, k0, k0 + NB, lds,
+                                tid);
+    const float* Cs = cfirst ? M.A : M.L;
+    const int64_t ldcs = cfirst ? M.lda : M.ld;
     for (int j0 = k0 + NB; j0 < n; j0 += 128)
-      stream_tiles<0, VEC, SPLIT>(M.L, M.ld, k0, j0, n, M.L, M.ld, j0, n, k0, M.L, Cs, M.ld, j0, n, lds, tid);
+      stream_tiles<0, VEC, SPLIT>(M.L, M.ld, k0, j0, n, M.L, M.ld, j0, n, k0, M.L, Cs, M.ld, ldcs, j0, n, lds, tid);
   }
-  if (p.first) zero_upper_one(M);
+  if (p.first || p.trailing_from_a) zero_upper_one(M);
 }
 
 // One workgroup per matrix (grid.x = the batch; workgroups are dispatched
@@ -1069,10 +1080,10 @@ int64_t diag_bulk_tiles(const Batch& p, const DiagBulk& u) {
 int launch_diag_factor(const Batch& p, int k0, float* winv, hipStream_t s, const DiagBulk* bulk,
                        const DiagZero* zero) {
   // The block that reaches column n needs no inverse (no panel follows).
-  // p.first (dense, k0 = 0): staged from A.  (Ragged batches: every matrix
-  // must end at this block for LAST; they never take p.first.)
+  // p.first (k0 = 0): staged from A, at A's own row stride.  (Ragged
+  // batches: every matrix must end at this block for LAST.)
   const bool last = k0 + kNB >= p.n;
-  const bool from_a = p.first && !p.Ap;
+  const bool from_a = p.first;
   const dim3 b(256);
   dim3 g((unsigned)p.batch);
   int zr0 = 0, zr1 = 0, nz = 0;

@@ -32,10 +32,11 @@ namespace {
 //   MODE 1: C  = D where i < ilim.
 // Every C load is issued unconditionally (a masked element reads C[0][0],
 // always valid) so the loads stay in flight together; only the stores are
-// masked.
+// masked.  The old values come from Cs with row stride ldcs (A's on a first
+// touch, which may differ from L's in a ragged batch).
 template <int FM, int FN, int MODE>
 __device__ __forceinline__ void epilogue(const floatx4 (&acc)[FM][FN], float* C, const float* Cs, int64_t ldc,
-                                         int r0, int c0, int ilim, int jlim, int li, int lg) {
+                                         int64_t ldcs, int r0, int c0, int ilim, int jlim, int li, int lg) {
   int64_t off[FM][FN][4];
   bool ok[FM][FN][4];
 #pragma unroll
@@ -57,7 +58,11 @@ __device__ __forceinline__ void epilogue(const floatx4 (&acc)[FM][FN], float* C,
 #pragma unroll
       for (int b = 0; b < FN; ++b)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) old[a][b][v] = gld(Cs + off[a][b][v]);
+        for (int v = 0; v < 4; ++v) {
+          const int i = r0 + a * 16 + 4 * lg + v;
+          const int j = c0 + b * 16 + li;
+          old[a][b][v] = gld(Cs + (ok[a][b][v] ? (int64_t)i * ldcs + j : 0));
+        }
 #pragma unroll
     for (int a = 0; a < FM; ++a)
 #pragma unroll
@@ -82,7 +87,8 @@ template <int BM, int BN, int MODE, bool VEC>
 __device__ __forceinline__ void gemm_tile(const float* Abase, int64_t lda, int a_row0, int a_lim,
                                           int ka, const float* Bbase, int64_t ldb, int b_row0,
                                           int b_lim, int kb, int K, float* Cbase, const float* Csrc,
-                                          int64_t ldc, int c_row0, int c_col0, int c_ilim, int c_jlim) {
+                                          int64_t ldc, int64_t ldcs, int c_row0, int c_col0, int c_ilim,
+                                          int c_jlim) {
   constexpr int WM = BM / 2, WN = BN / 2;      // per-wave tile
   constexpr int FM = WM / 16, FN = WN / 16;    // MFMA blocks per wave
   constexpr int PA = BM * KC / 4 / kThreads;
@@ -156,7 +162,7 @@ __device__ __forceinline__ void gemm_tile(const float* Abase, int64_t lda, int a
   }
 
   // Epilogue: lane holds D[4*lg + v][li] of each 16x16 block.
-  epilogue<FM, FN, MODE>(acc, Cbase, Csrc, ldc, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
+  epilogue<FM, FN, MODE>(acc, Cbase, Csrc, ldc, ldcs, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
 }
 
 // ------------------------------------------------------------ BF16x3 ----
@@ -172,8 +178,8 @@ __device__ __forceinline__ void gemm_tile(const float* Abase, int64_t lda, int a
 template <int BM, int BN, bool VEC, int MODE = 0>
 __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda, int a_row0, int a_lim, int ka,
                                                  const float* Bbase, int64_t ldb, int b_row0, int b_lim, int kb,
-                                                 int K, float* Cbase, const float* Csrc, int64_t ldc, int c_row0,
-                                                 int c_col0, int c_ilim, int c_jlim) {
+                                                 int K, float* Cbase, const float* Csrc, int64_t ldc,
+                                                 int64_t ldcs, int c_row0, int c_col0, int c_ilim, int c_jlim) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int PA = BM * KC / 4 / kThreads;
@@ -240,7 +246,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const float* Abase, int64_t lda
     __syncthreads();
   }
 
-  epilogue<FM, FN, MODE>(acc, Cbase, Csrc, ldc, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
+  epilogue<FM, FN, MODE>(acc, Cbase, Csrc, ldc, ldcs, c_row0 + wm, c_col0 + wn, c_ilim, c_jlim, li, lg);
 }
 
 // SYRK tile T x T (128, or 64 when a launch has too few 128-tiles to fill
@@ -258,13 +264,15 @@ __global__ __launch_bounds__(kThreads) void k_syrk_mfma(Batch p, int col0, int c
   int ti, tj;
   decode_tile(xcd_tile(blockIdx.x, (int)T64), TI, ti, tj);
   float* Lb = M.L;
-  const float* Cs = p.first ? M.A : M.L;
+  const float* Cs = p.first ? M.A : M.L;  // first touch: old values from A (its own row stride)
+  const int64_t ldcs = p.first ? M.lda : M.ld;
   const int i0 = col0 + T * ti;
   const int j0 = col0 + T * tj;
   if constexpr (SPLIT)
-    gemm_tile_bf16x3<T, T, VEC>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, kl, Lb, Cs, M.ld, i0, j0, M.n, c1);
+    gemm_tile_bf16x3<T, T, VEC>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, kl, Lb, Cs, M.ld, ldcs, i0, j0, M.n,
+                                c1);
   else
-    gemm_tile<T, T, 0, VEC>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, kl, Lb, Cs, M.ld, i0, j0, M.n, c1);
+    gemm_tile<T, T, 0, VEC>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, kl, Lb, Cs, M.ld, ldcs, i0, j0, M.n, c1);
 }
 
 // Panel: L21 = A21 W^T (SPLIT: BF16x3 products, the mixed form).
@@ -275,13 +283,15 @@ __global__ __launch_bounds__(kThreads) void k_panel_mfma(Batch p, int k0, const 
   const int i0 = k0 + kNB + PT * blockIdx.x;
   if (i0 >= M.n) return;  // (ragged) rows of a smaller matrix
   float* Lb = M.L;
-  const float* As = p.first ? M.A : M.L;
+  const float* As = p.first ? M.A : M.L;  // first touch: A21 from A (its own row stride)
+  const int64_t lda = p.first ? M.lda : M.ld;
   const float* Wb = winv + b * (kNB * kNB);
   if constexpr (SPLIT)
-    gemm_tile_bf16x3<PT, kNB, VEC, 1>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, Lb, M.ld, i0, k0, M.n,
-                                      k0 + kNB);
+    gemm_tile_bf16x3<PT, kNB, VEC, 1>(As, lda, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, Lb, M.ld, M.ld, i0, k0,
+                                      M.n, k0 + kNB);
   else
-    gemm_tile<PT, kNB, 1, VEC>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, Lb, M.ld, i0, k0, M.n, k0 + kNB);
+    gemm_tile<PT, kNB, 1, VEC>(As, lda, i0, M.n, k0, Wb, kNB, 0, kNB, 0, kNB, Lb, Lb, M.ld, M.ld, i0, k0, M.n,
+                               k0 + kNB);
 }
 
 // One-shot forms (K = 128) of the two kernels above (tiles in chol_once.h).
@@ -300,9 +310,10 @@ __global__ __launch_bounds__(kThreads) void k_panel_once(Batch p, int k0, const 
   if (i0 >= M.n) return;
   float* Lb = M.L;
   const float* As = p.first ? M.A : M.L;
+  const int64_t lda = p.first ? M.lda : M.ld;
   const float* Wb = winv + b * (kNB * kNB);
   __shared__ __align__(16) float lds[once_lds_floats<PT, kNB, SPLIT>()];
-  gemm_tile_once<PT, kNB, 1, VEC, SPLIT>(As, M.ld, i0, M.n, k0, Wb, kNB, 0, kNB, 0, Lb, Lb, M.ld, i0, k0, M.n,
+  gemm_tile_once<PT, kNB, 1, VEC, SPLIT>(As, lda, i0, M.n, k0, Wb, kNB, 0, kNB, 0, Lb, Lb, M.ld, M.ld, i0, k0, M.n,
                                          k0 + kNB, lds);
 }
 

@@ -102,15 +102,17 @@ constexpr int once_lds_floats() {
   return SPLIT ? (BM + BN) * K1H : (BM + BN) * K1P;
 }
 
-// MODE 0: C -= A B^T on the lower part (i >= j), i < ilim, j < jlim.
+// MODE 0: C -= A B^T on the lower part (i >= j), i < ilim, j < jlim, the old
+//         values read from Csrc with row stride ldcs (A's on a first touch);
 // MODE 1: C  = A B^T, i < ilim (panel; A and C alias row-for-row).
 // SPLIT: BF16x3 products (x = hi + lo, hi*hi + hi*lo + lo*hi).
 
 template <int BM, int BN, int MODE, bool VEC, bool SPLIT>
 __device__ __forceinline__ void gemm_tile_once(const float* Abase, int64_t lda, int a_row0, int a_lim, int ka,
                                                const float* Bbase, int64_t ldb, int b_row0, int b_lim, int kb,
-                                               float* Cbase, const float* Csrc, int64_t ldc, int c_row0,
-                                               int c_col0, int c_ilim, int c_jlim, float* __restrict__ lds) {
+                                               float* Cbase, const float* Csrc, int64_t ldc, int64_t ldcs,
+                                               int c_row0, int c_col0, int c_ilim, int c_jlim,
+                                               float* __restrict__ lds) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int PA = BM * K1 / 4 / kThreads;
@@ -146,7 +148,7 @@ __device__ __forceinline__ void gemm_tile_once(const float* Abase, int64_t lda, 
         const bool m = (MODE == 0) ? (i < c_ilim && j < c_jlim && j <= i) : (i < c_ilim);
         ok[a][b][v] = m;
         off[a][b][v] = m ? (int64_t)i * ldc + j : 0;
-        if constexpr (MODE == 0) old[a][b][v] = gld(Csrc + off[a][b][v]);
+        if constexpr (MODE == 0) old[a][b][v] = gld(Csrc + (m ? (int64_t)i * ldcs + j : 0));
       }
 
   floatx4 acc[FM][FN];
@@ -261,8 +263,8 @@ __device__ __forceinline__ void syrk_once_tile(const Batch& p, int bid, int64_t 
   const float* Cs = first ? M.A : M.L;
   const int i0 = col0 + T * ti;
   const int j0 = col0 + T * tj;
-  gemm_tile_once<T, T, 0, VEC, SPLIT>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, Lb, Cs, M.ld, i0, j0, M.n, c1,
-                                      lds);
+  gemm_tile_once<T, T, 0, VEC, SPLIT>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, Lb, Cs, M.ld,
+                                      first ? M.lda : M.ld, i0, j0, M.n, c1, lds);
 }
 
 // Rectangular one-shot tiles of the same update, TM rows x TN columns
@@ -300,8 +302,8 @@ __device__ __forceinline__ void syrk_once_rect(const Batch& p, int bid, int64_t 
   if (skip_diag && i0 < col0 + 128 && j0 < col0 + 128) return;
   float* Lb = M.L;
   const float* Cs = first ? M.A : M.L;
-  gemm_tile_once<TM, TN, 0, VEC, SPLIT>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, Lb, Cs, M.ld, i0, j0, M.n, c1,
-                                        lds);
+  gemm_tile_once<TM, TN, 0, VEC, SPLIT>(Lb, M.ld, i0, M.n, k0, Lb, M.ld, j0, c1, k0, Lb, Cs, M.ld,
+                                        first ? M.lda : M.ld, i0, j0, M.n, c1, lds);
 }
 }  // namespace
 }  // namespace chol

@@ -3,9 +3,11 @@
 //
 // The host sorts the matrices by order (largest first) and uploads one
 // descriptor table (pointers, orders, leading dimensions of L, original
-// positions) into the workspace.  Every matrix above order 64 is copied
-// first (L = tril(A), k_copy_lower).  The sorted list then splits into
-// three size classes, each one launch sequence over its contiguous slice:
+// positions) into the workspace.  Only each matrix's first 128 columns are
+// copied into L (k_copy_head); the trailing updates that first touch the
+// rest read its old values from A, at A's own row stride, and the upper
+// triangle beyond is zeroed by rows.  The sorted list splits into three
+// size classes, each one launch sequence over its contiguous slice:
 //   n <= 64 : persistent wavefront kernel (k_potrf_small_queue), one
 //             launch per padded order 64/32/16, fed by an atomic work queue
 //             over the sorted slice;
@@ -13,8 +15,8 @@
 //             one launch of the fused kernel (k_potrf_fused), a workgroup
 //             per matrix that runs its whole factorisation, the largest
 //             matrices dispatched first;
-//   larger  : the blocked path in lock
-//             step: outer blocks of nbo columns (the tuning table's width
+//   larger  : the zero fill of L's upper triangles, then the blocked path
+//             in lock step: outer blocks of nbo columns (the tuning table's width
 //             for the largest order, else 256, 512 from 1024); inside one,
 //             128-column steps whose diagonal-block launch touches the
 //             matrices with n > k0 (a prefix of the sorted list) and whose
@@ -24,10 +26,10 @@
 //             matrix's columns beyond the block.  Each kernel reads its
 //             matrix's order and leading dimension from the table (blocks
 //             past a smaller matrix exit).  Rows of L padded to a multiple
-//             of 4 (ld_host) give every kernel after the copy float4
-//             access whatever the orders.
-// The first two classes run on the library's side stream, beside the lock
-// step on the caller's stream.
+//             of 4 (ld_host) give every kernel float4 access to L whatever
+//             the orders.
+// All but the lock step run on the library's side stream, beside it on the
+// caller's stream.
 // The reference factors one fixed (batch, n) per call (custom_kernel,
 // submission.py:4101-4268); this generalises its per-shape dispatch.
 #include <algorithm>
@@ -186,21 +188,31 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
     return q;
   };
   float* const winv0 = reinterpret_cast<float*>(w + lay.winv);
-
-  // L = tril(A) for every matrix above 64, in chunks of at most 65535 (the
-  // copy puts the matrix on grid.y).
+  // No copy of tril(A) into L beyond each matrix's first 128 columns
+  // (k_copy_head), which the first diagonal block and panel read from L's
+  // aligned rows (A's rows, at stride n, are unaligned for odd n); every
+  // later region is first touched by a trailing update, which reads its old
+  // values from A at A's own row stride, and the rest of the upper triangle
+  // is zeroed by rows.  (The matrix on grid.y: chunks of at most 65535.)
   for (int64_t c0 = 0; c0 < c64; c0 += 65535)
-    if ((rc = launch_copy_lower(slice(c0, std::min<int64_t>(c64 - c0, 65535)), s))) return rc;
+    if ((rc = launch_copy_head(slice(c0, std::min<int64_t>(c64 - c0, 65535)), s))) return rc;
 
-  // Mid orders [cF, c64): one fused launch, a workgroup per matrix,
-  // largest first; then the n <= 64 queues.  Both on the side stream when
-  // the lock-step class below has work for the caller's stream, so the
-  // classes overlap.
+  // Side stream (when the lock-step class below has work for the caller's
+  // stream, so the classes overlap): the zero fill of the lock-step
+  // matrices (no kernel of theirs reads there), the fused launch of the mid
+  // orders [cF, c64) (a workgroup per matrix, largest first), then the
+  // n <= 64 queues.
   const bool split = mode == CHOL_MODE_MIXED;
   hipStream_t s2 = s;
   std::unique_lock<std::mutex> side_lock;
   if (cF > 0 && cz > cF && (rc = side_begin(s, &s2, side_lock))) return rc;
-  if (c64 > cF && (rc = launch_fused(slice(cF, c64 - cF), winv0 + cF * kNB * kNB, split, s2))) return rc;
+  for (int64_t c0 = 0; c0 < cF; c0 += 65535)
+    if ((rc = launch_zero_upper_rows(slice(c0, std::min<int64_t>(cF - c0, 65535)), s2))) return rc;
+  if (c64 > cF) {
+    Batch q = slice(cF, c64 - cF);
+    q.trailing_from_a = true;
+    if ((rc = launch_fused(q, winv0 + cF * kNB * kNB, split, s2))) return rc;
+  }
   // n <= 64: persistent wavefronts (4 per workgroup, 64/np matrices each).
   auto waves = [](int64_t cnt, int np) { return (cnt * np + 63) / 64; };
   if ((rc = launch_small_queue(base, c64, c32 - c64, 64, queues + 2,
@@ -219,6 +231,8 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
     const int64_t cnt = std::min<int64_t>(cF - c0, 65535);
     const int32_t* cns = ns + c0;  // host copy of the chunk's orders (sorted, largest first)
     Batch q = slice(c0, cnt);
+    Batch qf = q;  // first-touch updates (outer block 0: old values from A)
+    qf.first = true;
     float* winv = winv0 + c0 * kNB * kNB;
     auto active = [&](int64_t above) {  // matrices of the chunk with n > above
       int64_t a = 0;
@@ -239,20 +253,26 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
       // updates are bound by the matrix cores, 4226 -> 4565 us, so it
       // stays right-looking.
       const bool left = split && (int64_t)((cns[0] - K0) / kNB) * active(K0) >= 256;
+      // First touch in the first outer block (columns from 128 on): each
+      // column block's left-looking update, the step-0 right-looking update
+      // of the block's later columns, and the update of every column beyond
+      // the block.
+      Batch& q0 = K0 == 0 ? qf : q;
       for (int k0 = K0; k0 < E && k0 < cns[0]; k0 += kNB) {
-        q.batch = active(k0);
-        if (left && k0 > K0 && (rc = launch_syrk(q, k0, k0 + kNB, K0, k0 - K0, split, s))) return rc;
+        Batch& qk = k0 == 0 ? qf : q;
+        q.batch = qf.batch = active(k0);
+        if (left && k0 > K0 && (rc = launch_syrk(q0, k0, k0 + kNB, K0, k0 - K0, split, s))) return rc;
         if ((rc = launch_diag_factor(q, k0, winv, s))) return rc;
         const int64_t act2 = active(k0 + kNB);
         if (act2 == 0) break;
-        q.batch = act2;
+        q.batch = qf.batch = act2;
         if ((rc = launch_panel(q, k0, winv, split, s))) return rc;
-        if (!left && k0 + kNB < E && (rc = launch_syrk(q, k0 + kNB, E, k0, kNB, split, s))) return rc;
+        if (!left && k0 + kNB < E && (rc = launch_syrk(qk, k0 + kNB, E, k0, kNB, split, s))) return rc;
       }
       const int64_t actE = active(E);
       if (actE == 0) break;
-      q.batch = actE;
-      if ((rc = launch_syrk(q, E, -1, K0, (int)nbo, split, s))) return rc;
+      q.batch = qf.batch = actE;
+      if ((rc = launch_syrk(q0, E, -1, K0, (int)nbo, split, s))) return rc;
     }
   }
   if ((rc = side_end(s, s2))) return rc;

@@ -124,6 +124,7 @@ __global__ __launch_bounds__(NT) void k_syrk256(Batch p, int col0, int col1, int
   float* L = M.L;
   const float* Cs = p.first ? M.A : M.L;  // old values of C (first touch: A)
   const int64_t ld = M.ld;
+  const int64_t ldcs = p.first ? M.lda : M.ld;  // row stride of Cs
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -232,12 +233,12 @@ __global__ __launch_bounds__(NT) void k_syrk256(Batch p, int col0, int col1, int
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = i0 + wm + (h * (FM / 2) + a) * 16 + 4 * lg + v;
-        const float* row = Cs + (int64_t)(i < M.n ? i : i0) * ld;
+        const float* row = Cs + (int64_t)(i < M.n ? i : i0) * ldcs;
 #pragma unroll
         for (int c = 0; c < FN; ++c) {
           const int j = jl + c * 16;
           const bool m = i < M.n && j < c1 && j <= i;
-          old[a][c][v] = gld(m ? row + j : Cs + (int64_t)i0 * ld + j0);
+          old[a][c][v] = gld(m ? row + j : Cs + (int64_t)i0 * ldcs + j0);
         }
       }
 #pragma unroll
