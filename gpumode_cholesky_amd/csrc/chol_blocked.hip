@@ -31,72 +31,6 @@ namespace chol {
 namespace {
 
 // ---------------------------------------------------------------- copy ----
-// L = tril(A) with exact zeros above the diagonal.  Workgroup x takes rows
-// x, x + gridDim.x, ... of matrix blockIdx.y (any matrix of a ragged
-// table); its threads run along the row.  A row of L that starts 16-byte
-// aligned (rows padded to a multiple of 4) is written in float4 chunks,
-// read from A as float4 where A's row is aligned too and as scalars where
-// not, the last n % 4 elements one by one.  Other rows move in float4
-// chunks between a scalar head and tail where A's and L's rows share their
-// 16-byte offset, element by element otherwise.
-__global__ __launch_bounds__(256) void k_copy_lower(Batch p) {
-  const Mat M = mat_at(p, blockIdx.y);
-  const int n = M.n;
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    const float* src = M.A + (int64_t)i * M.lda;
-    float* dst = M.L + (int64_t)i * M.ld;
-    const int t = threadIdx.x;
-    if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
-      const bool svec = (reinterpret_cast<uintptr_t>(src) & 15) == 0;
-      const int n4 = n >> 2;
-      for (int c = t; c < n4; c += blockDim.x) {
-        const int jj = 4 * c;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (jj <= i) {
-          if (svec) {
-            v = gld4_nt(src + jj);  // the chunk lies inside the row
-          } else {
-            v.x = gld_nt(src + jj);
-            if (jj + 1 <= i) v.y = gld_nt(src + jj + 1);
-            if (jj + 2 <= i) v.z = gld_nt(src + jj + 2);
-            if (jj + 3 <= i) v.w = gld_nt(src + jj + 3);
-          }
-        }
-        if (jj + 1 > i) v.y = 0.f;
-        if (jj + 2 > i) v.z = 0.f;
-        if (jj + 3 > i) v.w = 0.f;
-        gst4(dst + jj, v);
-      }
-      const int j = 4 * n4 + t;
-      if (t < (n & 3)) gst(dst + j, (j <= i) ? gld_nt(src + j) : 0.f);
-      continue;
-    }
-    const int head = (int)((4 - ((reinterpret_cast<uintptr_t>(src) >> 2) & 3)) & 3);
-    const bool vec = ((reinterpret_cast<uintptr_t>(src) ^ reinterpret_cast<uintptr_t>(dst)) & 15) == 0 && head < n;
-    if (vec) {
-      const int n4 = (n - head) >> 2;
-      const int t0 = head + 4 * n4;  // first element of the tail
-      int j = -1;
-      if (t < head)
-        j = t;
-      else if (t >= 64 && t - 64 < n - t0)
-        j = t0 + t - 64;
-      if (j >= 0) gst(dst + j, (j <= i) ? gld_nt(src + j) : 0.f);
-      for (int c = t; c < n4; c += blockDim.x) {
-        const int jj = head + 4 * c;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (jj <= i) v = gld4_nt(src + jj);
-        if (jj + 1 > i) v.y = 0.f;
-        if (jj + 2 > i) v.z = 0.f;
-        if (jj + 3 > i) v.w = 0.f;
-        gst4(dst + jj, v);
-      }
-    } else {
-      for (int jj = t; jj < n; jj += blockDim.x) gst(dst + jj, (jj <= i) ? gld_nt(src + jj) : 0.f);
-    }
-  }
-}
-
 // Ragged first touch, head part: L[i][j] = (j <= i ? A[i][j] : 0) for the
 // first min(n, 128) columns of every row (the first column block, which the
 // first diagonal block and panel read, so that they read L's aligned rows;
@@ -172,22 +106,6 @@ int launch_zero_info(int32_t* info, int64_t batch, hipStream_t s) {
   if (!info || batch <= 0) return 0;
   hipLaunchKernelGGL(k_zero_info, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, s, info, batch);
   return launch_status("k_zero_info");
-}
-
-// Rows per matrix for k_copy_lower: about 8 KiB of one matrix's traffic
-// per workgroup, so that a workgroup does several rows of a large matrix
-// (grid.x is set by the largest matrix of a ragged table; rows past a
-// smaller one exit).
-static unsigned row_groups(int64_t n) {
-  int64_t gx = n * n / 8192;
-  if (gx < 1) gx = 1;
-  if (gx > n) gx = n;
-  return (unsigned)gx;
-}
-
-int launch_copy_lower(const Batch& p, hipStream_t s) {
-  hipLaunchKernelGGL(k_copy_lower, dim3(row_groups(p.n), (unsigned)p.batch), dim3(256), 0, s, p);
-  return launch_status("k_copy_lower");
 }
 
 int launch_copy_head(const Batch& p, hipStream_t s) {

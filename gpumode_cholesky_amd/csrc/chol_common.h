@@ -265,7 +265,6 @@ inline int launch_status(const char* what) {
 // Launchers implemented in the kernel translation units.
 int launch_small(const Batch& p, hipStream_t s);                  // n <= 64
 int launch_lds(const Batch& p, hipStream_t s);                    // n <= 256
-int launch_copy_lower(const Batch& p, hipStream_t s);             // L = tril(A)
 int launch_copy_head(const Batch& p, hipStream_t s);              // the first 128 columns of L = tril(A)
 int launch_zero_upper_rows(const Batch& p, hipStream_t s);        // L = 0 above the diagonal beyond them
 int launch_zero_upper(const Batch& p, hipStream_t s);             // L = 0 right of the diagonal chunks
