@@ -17,7 +17,7 @@
 //             matrices dispatched first;
 //   larger  : the zero fill of L's upper triangles, then the blocked path
 //             in lock step: outer blocks of nbo columns (the tuning table's width
-//             for the largest order, else 256, 512 from 1024); inside one,
+//             for the largest order, else 256, 512 above 1024); inside one,
 //             128-column steps whose diagonal-block launch touches the
 //             matrices with n > k0 (a prefix of the sorted list) and whose
 //             panel and right-looking K = 128 update touch those with
@@ -127,8 +127,9 @@ int chol_potrf_ragged_f32(const float* const* A_host, float* const* L_host, cons
   auto* ns = reinterpret_cast<int32_t*>(desc.data() + (lay.ns - lay.Ap));
   auto* orig = reinterpret_cast<int32_t*>(desc.data() + (lay.orig - lay.Ap));
   auto* lds = reinterpret_cast<int32_t*>(desc.data() + (lay.lds - lay.Ap));
-  // float4 access by every kernel after the copy: each L row of a matrix
-  // with n > 64 starts 16-byte aligned (A is read by the copy alone)
+  // float4 access to L by every kernel: each L row of a matrix with n > 64
+  // starts 16-byte aligned (A, whose rows are n apart, is read element by
+  // element: by the head copy and by the first-touch updates' old values)
   bool aligned = true;
   for (int64_t i = 0; i < count; ++i) {
     const int64_t j = ord[i];
