@@ -118,6 +118,22 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The BF16x3 operand split of a pair of FP32 values, two to a dword:
+// hi = x rounded to BF16 (nearest even; one v_cvt_pk_bf16_f32 for the
+// pair), lo = the exact FP32 remainder x - hi rounded to BF16.  hi's FP32
+// values come from the packed dword by a shift and a mask, so a pair costs
+// six VALU operations (converting each hi on its own and packing it again
+// cost eight).  Element 0 in the low half.
+__device__ __forceinline__ void split_bf16_pair(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+  typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2){x0, x1}, bf2));
+  const float h0 = __builtin_bit_cast(float, u << 16);
+  const float h1 = __builtin_bit_cast(float, u & 0xffff0000u);
+  hi = u;
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2){x0 - h0, x1 - h1}, bf2));
+}
+
 // Right-looking factorisation of the leading NP columns of a block held one
 // row per lane: a[] holds columns 0..NP-1 of the lane's row r (lower part;
 // the upper part is never read by other lanes), cb the per-block LDS column

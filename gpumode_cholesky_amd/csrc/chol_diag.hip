@@ -51,7 +51,6 @@ __device__ __forceinline__ floatx4 mma_nn(const float* X, int px, const float* Y
 
 constexpr int XP = 16 + 4;  // pitch of the L21 panel slabs
 constexpr int PH = NB + 8;  // BF16 pitch of a staged previous-panel block (PRE)
-typedef __bf16 bf16x4d __attribute__((ext_vector_type(4)));
 
 // LDS of the kernel: the staging tiles first, then the block and its
 // inverse (Ls starts below 64 KiB, so most of its accesses encode their
@@ -515,12 +514,11 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
   auto put_p = [&](int u) {  // P row tid/32 + 8u as BF16 hi / lo
     const int r = (tid >> 5) + 8 * u;
     const float4 v = pst[u];
-    const __bf16 h0 = (__bf16)v.x, h1 = (__bf16)v.y, h2 = (__bf16)v.z, h3 = (__bf16)v.w;
-    const bf16x4d hi = {h0, h1, h2, h3};
-    const bf16x4d lo = {(__bf16)(v.x - (float)h0), (__bf16)(v.y - (float)h1), (__bf16)(v.z - (float)h2),
-                        (__bf16)(v.w - (float)h3)};
-    *reinterpret_cast<bf16x4d*>(Ph + r * PH + c4) = hi;
-    *reinterpret_cast<bf16x4d*>(Ph + NB * PH + r * PH + c4) = lo;
+    uint2 hi, lo;
+    split_bf16_pair(v.x, v.y, hi.x, lo.x);
+    split_bf16_pair(v.z, v.w, hi.y, lo.y);
+    *reinterpret_cast<uint2*>(Ph + r * PH + c4) = hi;
+    *reinterpret_cast<uint2*>(Ph + NB * PH + r * PH + c4) = lo;
   };
 #pragma unroll
   for (int u = 0; u < NB / 8; ++u) {

@@ -59,7 +59,6 @@ __device__ __forceinline__ void store_rows(float* __restrict__ lds, const float4
 }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 constexpr int KPH = KC + 8;  // bf16 LDS row pitch (80 bytes)
 
 template <int ROWS, int KW = KC, int PITCH = KPH>
@@ -72,15 +71,15 @@ __device__ __forceinline__ void store_rows_split(__bf16* __restrict__ hi, __bf16
     const int r = c / (KW / 4);
     const int q = (c % (KW / 4)) * 4;
     const float4 v = reg[u];
-    const __bf16 h0 = (__bf16)v.x, h1 = (__bf16)v.y, h2 = (__bf16)v.z, h3 = (__bf16)v.w;
-    const __bf16 l0 = (__bf16)(v.x - (float)h0), l1 = (__bf16)(v.y - (float)h1);
-    const __bf16 l2 = (__bf16)(v.z - (float)h2), l3 = (__bf16)(v.w - (float)h3);
-    bf16x2* ph = reinterpret_cast<bf16x2*>(hi + r * PITCH + q);
-    bf16x2* pl = reinterpret_cast<bf16x2*>(lo + r * PITCH + q);
-    ph[0] = bf16x2{h0, h1};
-    ph[1] = bf16x2{h2, h3};
-    pl[0] = bf16x2{l0, l1};
-    pl[1] = bf16x2{l2, l3};
+    uint32_t h01, l01, h23, l23;
+    split_bf16_pair(v.x, v.y, h01, l01);
+    split_bf16_pair(v.z, v.w, h23, l23);
+    uint32_t* ph = reinterpret_cast<uint32_t*>(hi + r * PITCH + q);
+    uint32_t* pl = reinterpret_cast<uint32_t*>(lo + r * PITCH + q);
+    ph[0] = h01;
+    ph[1] = h23;
+    pl[0] = l01;
+    pl[1] = l23;
   }
 }
 

@@ -36,7 +36,6 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int BT = 256;                  // tile edge
 constexpr int NT = 512;                  // threads (8 wavefronts)
@@ -92,13 +91,12 @@ __device__ __forceinline__ void put_split(__bf16* __restrict__ s, const float4 (
     const int r = c >> 3;
     const int q = c & 7;
     const float4 v = reg[u];
-    const __bf16 h0 = (__bf16)v.x, h1 = (__bf16)v.y, h2 = (__bf16)v.z, h3 = (__bf16)v.w;
-    const bf16x4 hi = {h0, h1, h2, h3};
-    const bf16x4 lo = {(__bf16)(v.x - (float)h0), (__bf16)(v.y - (float)h1), (__bf16)(v.z - (float)h2),
-                       (__bf16)(v.w - (float)h3)};
+    uint2 hi, lo;
+    split_bf16_pair(v.x, v.y, hi.x, lo.x);
+    split_bf16_pair(v.z, v.w, hi.y, lo.y);
     const int off = r * KC + (((q >> 1) ^ ((r >> 1) & 3)) << 3) + ((q & 1) << 2);
-    *reinterpret_cast<bf16x4*>(s + off) = hi;
-    *reinterpret_cast<bf16x4*>(s + BT * KC + off) = lo;
+    *reinterpret_cast<uint2*>(s + off) = hi;
+    *reinterpret_cast<uint2*>(s + BT * KC + off) = lo;
   }
 }
 
