@@ -180,6 +180,12 @@ __global__ __launch_bounds__(NT) void k_syrk256(Batch p, int col0, int col1, int
       }
 #pragma unroll
       for (int a = 0; a < FM; ++a) {
+        // The next chunk's split and LDS stores go in halfway through this
+        // chunk's products, overlapping the matrix cores: its buffer is not
+        // read in this chunk (every wave passed the last barrier after
+        // reading it).  Staged after the products instead: 3 % slower
+        // alone, 1-1.5 % on 1 x 16384 / 1 x 32768 calls.
+        if (a == FM / 2 && more) stage(cur ^ 1);
         const int o = (wm + a * 16 + li) * KC + sl;
         const bf16x8 ah = *reinterpret_cast<const bf16x8*>(Ah + o);
         const bf16x8 al = *reinterpret_cast<const bf16x8*>(Al + o);
@@ -213,7 +219,7 @@ __global__ __launch_bounds__(NT) void k_syrk256(Batch p, int col0, int col1, int
         }
       }
     }
-    if (more) stage(cur ^ 1);
+    if (!SPLIT && more) stage(cur ^ 1);  // FP32: after the products (halfway: 1-2 % slower)
     __syncthreads();
   }
 
