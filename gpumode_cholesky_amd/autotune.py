@@ -61,24 +61,41 @@ def candidates(n: int):
     return out
 
 
-def time_config(a: torch.Tensor, mode: str, reps: int = 5) -> float:
-    """Median microseconds per call over ``reps`` event-timed calls."""
+def time_once(a: torch.Tensor, mode: str) -> float:
+    """Microseconds of one event-timed call (caches flushed first)."""
     from .cholesky import cholesky
     from .harness import flush_caches
 
+    flush_caches(a.device)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
     cholesky(a, mode=mode)
+    e1.record()
     torch.cuda.synchronize()
-    ts = []
-    for _ in range(reps):
-        flush_caches(a.device)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
+    return e0.elapsed_time(e1) * 1e3
+
+
+def time_candidates(a: torch.Tensor, mode: str, cands, reps: int = 5) -> dict:
+    """Median microseconds per call of every candidate table value for the
+    shape of ``a``.  Each is warmed up first; then the timed calls go
+    round-robin over the candidates, the order reversed every round, so a
+    drift of the GPU's clock or memory state over the run (the first
+    calls of a process ran a few per cent slower) weighs on all of them
+    alike instead of on whichever is timed first."""
+    from .cholesky import cholesky
+
+    batch, n = a.shape[0], a.shape[-1]
+    for nbo in cands:
+        set_table([[n, n, batch, batch, nbo]])
         cholesky(a, mode=mode)
-        e1.record()
-        torch.cuda.synchronize()
-        ts.append(e0.elapsed_time(e1) * 1e3)
-    return float(np.median(ts))
+    torch.cuda.synchronize()
+    ts = {nbo: [] for nbo in cands}
+    for r in range(reps):
+        for nbo in (cands if r % 2 == 0 else cands[::-1]):
+            set_table([[n, n, batch, batch, nbo]])
+            ts[nbo].append(time_once(a, mode))
+    return {nbo: float(np.median(v)) for nbo, v in ts.items()}
 
 
 def tune(shapes, mode: str = "plain", reps: int = 5, log=print) -> list:
@@ -93,13 +110,10 @@ def tune(shapes, mode: str = "plain", reps: int = 5, log=print) -> list:
         if n <= 64:
             continue
         a = config_input(batch, n, 1234 + n, device=dev)
-        best = None
-        for nbo in candidates(n):
-            set_table([[n, n, batch, batch, nbo]])
-            us = time_config(a, mode, reps)
+        times = time_candidates(a, mode, candidates(n), reps)
+        for nbo, us in times.items():
             log(f"tune {batch}x{n} {mode} nbo={nbo}: {us:.1f} us")
-            if best is None or us < best[1]:
-                best = (nbo, us)
+        best = min(times.items(), key=lambda kv: kv[1])
         rows.append([n, n, batch, batch, mcode, best[0]])
         log(f"tune {batch}x{n} {mode}: best nbo={best[0]} ({best[1]:.1f} us)")
         del a
