@@ -604,7 +604,15 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
     w_block_row<2>(Ls, Ws, Ts, 0, lane, li, lg);
   }
   __syncthreads();
-  if (wave == 1 || wave == 2) CHOL_L21_ROUND(3)
+  if (wave == 1 || wave == 2) {
+    CHOL_L21_ROUND(3)
+  } else if constexpr (!LAST) {
+    // wave 3, otherwise idle here: the sums of W1's block row 3 (its
+    // operands, D1's rows 48..63 and W1's rows 0..47, are final); the
+    // rest of that block row follows in the next round, which it used to
+    // hold up
+    if (wave == 3) w_row_sums<3>(Ls, Ws, Ts, 0, li, lg);
+  }
 #undef CHOL_L21_ROUND
   __syncthreads();
 
@@ -694,7 +702,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
     }
     if constexpr (!LAST) store_w16(32);
   } else if constexpr (!LAST) {
-    w_block_row<3>(Ls, Ws, Ts, 0, lane, li, lg);
+    w_row_finish<3>(Ls, Ws, Ts, 0, lane, li, lg);  // its sums were formed in the round before
     store_w16(48);
   }
   if constexpr (!LAST) __syncthreads();
