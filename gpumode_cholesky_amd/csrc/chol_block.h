@@ -1,7 +1,7 @@
 // chol_block.h — the 64x64 block factorisation shared by the diagonal-block
 // kernel (chol_diag.hip) and the n = 64 whole-matrix kernel (chol_small.hip):
-// 16-column register panels with v_readlane pivot broadcasts and MFMA
-// trailing updates, on a block resident in LDS.
+// 16-column register panels with v_readlane pivot broadcasts, each brought
+// up to date by the earlier panels on MFMA first, on a block resident in LDS.
 #pragma once
 
 #include "chol_common.h"
@@ -24,12 +24,53 @@ __device__ __forceinline__ floatx4_b mma_nt_p(const float* X, const float* Y, fl
 }
 
 // One column panel S of factor64 (below): the register chain on rows
-// c0..d+63 (c0 = d + PW S) and the panel's trailing update.  Accumulates the
-// 1-based failing pivot in bad.
-template <int PW, int P, int S>
+// c0..d+63 (c0 = d + PW S) and the MFMA updates around it.  Panels before
+// LF update right-looking: after its chain, panel S < LF applies its rank-PW
+// update to all later columns at once.  From LF on, updates are
+// left-looking: before its chain, panel S takes the updates of panels
+// LF..S-1 on its own columns only (K = PW (S - LF), one MFMA sum and one
+// read-modify-write per 16x16 tile).
+//   LF = 0 (all left-looking): each tile is written once in all and panel 0
+//     updates nothing, where a right-looking panel 0 updates six tiles: the
+//     shorter path for a wavefront that factors alone, or whose partners
+//     are idle in the first panel.
+//   LF = 1: panel 0's update of all later columns fills the first panel,
+//     in which the diagonal-block kernel's other wavefronts are busy, and
+//     panel 1's update moves out of the second panel into the last two.
+//   LF = 4: all right-looking.
+// Accumulates the 1-based failing pivot in bad.
+template <int PW, int P, int S, int LF>
 __device__ __forceinline__ void factor64_step(float* Ls, int d, int lane, int& bad) {
   const int li = lane & 15, lg = lane >> 4;
   const int c0 = d + PW * S;
+  if constexpr (S > LF) {
+    constexpr int t0 = PW * S / 16;  // the panel's first tile row
+    constexpr int NT = 4 - t0;       // its tiles (t0 + t, t0), t < NT
+    floatx4_b acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = floatx4_b{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = PW * LF; kk < PW * S; kk += 16)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        acc[t] = mma_nt_p<P>(&Ls[(d + 16 * (t0 + t)) * P + d + kk], &Ls[c0 * P + d + kk], acc[t], li, lg);
+    // all old values first, then every element written back (the diagonal
+    // tile's upper part unchanged): no branch, one LDS round trip.  The
+    // block is this wavefront's alone while it factors it.
+    float old[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) old[t][v] = Ls[(d + 16 * (t0 + t) + 4 * lg + v) * P + c0 + li];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * t + 4 * lg + v;
+        Ls[(d + 16 * t0 + i) * P + c0 + li] = (t != 0 || li <= i) ? old[t][v] - acc[t][v] : old[t][v];
+      }
+    wave_lds_sync();
+  }
   float* row = &Ls[(d + lane) * P + c0];
   float a[PW];
 #pragma unroll
@@ -48,40 +89,37 @@ __device__ __forceinline__ void factor64_step(float* Ls, int d, int lane, int& b
       *reinterpret_cast<float4*>(row + j) = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
   }
   wave_lds_sync();
-  // trailing update: lower 16x16 tiles (ti >= tj) of rows / columns
+  // right-looking: lower 16x16 tiles (ti >= tj) of rows / columns
   // [PW (S+1), 64); reads only panel columns, writes only later ones
-  constexpr int t0 = PW * (S + 1) / 16;
-  if constexpr (t0 < 4) {
+  constexpr int t1 = PW * (S + 1) / 16;
+  if constexpr (S < LF && t1 < 4) {
     floatx4_b acc[10];
     int nt = 0;
 #pragma unroll
-    for (int ti = t0; ti < 4; ++ti)
+    for (int ti = t1; ti < 4; ++ti)
 #pragma unroll
-      for (int tj = t0; tj <= ti; ++tj) {
+      for (int tj = t1; tj <= ti; ++tj) {
         floatx4_b c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kk = 0; kk < PW; kk += 16)
           c = mma_nt_p<P>(&Ls[(d + 16 * ti) * P + c0 + kk], &Ls[(d + 16 * tj) * P + c0 + kk], c, li, lg);
         acc[nt++] = c;
       }
-    // all old values first, then every element written back (a diagonal
-    // tile's upper part unchanged): no branch, one LDS round trip.  The
-    // block is this wavefront's alone while it factors it.
     float old[10][4];
     nt = 0;
 #pragma unroll
-    for (int ti = t0; ti < 4; ++ti)
+    for (int ti = t1; ti < 4; ++ti)
 #pragma unroll
-      for (int tj = t0; tj <= ti; ++tj) {
+      for (int tj = t1; tj <= ti; ++tj) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) old[nt][v] = Ls[(d + 16 * ti + 4 * lg + v) * P + d + 16 * tj + li];
         ++nt;
       }
     nt = 0;
 #pragma unroll
-    for (int ti = t0; ti < 4; ++ti)
+    for (int ti = t1; ti < 4; ++ti)
 #pragma unroll
-      for (int tj = t0; tj <= ti; ++tj) {
+      for (int tj = t1; tj <= ti; ++tj) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int i = 16 * ti + 4 * lg + v, j = 16 * tj + li;
@@ -95,11 +133,11 @@ __device__ __forceinline__ void factor64_step(float* Ls, int d, int lane, int& b
 
 // One wavefront factors the 64x64 block of Ls (pitch P) at (d, d), blocked
 // by PW columns: for each column panel [c0, c0+PW), lane = row,
-//   1. reg_factor_rl<PW> on rows c0..d+63 yields the panel's diagonal factor
+//   1. the panel's lower 16x16 tiles take the update of all earlier panels
+//      on v_mfma_f32_16x16x4_f32 (left-looking);
+//   2. reg_factor_rl<PW> on rows c0..d+63 yields the panel's diagonal factor
 //      and, for the rows below it, its part of L (the panel's own
-//      triangular solve), broadcasting pivot columns with v_readlane;
-//   2. the lower 16x16 tiles of the trailing block take the panel's rank-PW
-//      update on v_mfma_f32_16x16x4_f32.
+//      triangular solve), broadcasting pivot columns with v_readlane.
 // A pivot step then costs a few dependent VALU operations plus
 // 2 (PW-1-k) issue slots, where a 64-wide register chain spends an LDS
 // round trip and 63-k row updates.  Returns the 1-based first failing
@@ -108,10 +146,10 @@ template <int PW, int P>
 __device__ __forceinline__ int factor64(float* Ls, int d, int lane) {
   static_assert(PW == 16, "four 16-column panels");
   int bad = 0;
-  factor64_step<PW, P, 0>(Ls, d, lane, bad);
-  factor64_step<PW, P, 1>(Ls, d, lane, bad);
-  factor64_step<PW, P, 2>(Ls, d, lane, bad);
-  factor64_step<PW, P, 3>(Ls, d, lane, bad);
+  factor64_step<PW, P, 0, 0>(Ls, d, lane, bad);
+  factor64_step<PW, P, 1, 0>(Ls, d, lane, bad);
+  factor64_step<PW, P, 2, 0>(Ls, d, lane, bad);
+  factor64_step<PW, P, 3, 0>(Ls, d, lane, bad);
   return bad;
 }
 

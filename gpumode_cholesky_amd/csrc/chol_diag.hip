@@ -3,9 +3,9 @@
 //
 // One workgroup of 4 wavefronts per matrix, the block resident in LDS.
 // Wavefront 0 runs the pivot chains: the 64x64 blocks D1 (top left) and then
-// D2 (bottom right), each as four 16-column panels (factor64_step: register
-// chain, lane = row, pivots broadcast with v_readlane, MFMA update of the
-// panel's trailing tiles).  Everything else runs on wavefronts 1-3 in the
+// D2 (bottom right), each as four 16-column panels (factor64_step: MFMA
+// update of the panel by the earlier ones, then the register chain, lane =
+// row, pivots broadcast with v_readlane).  Everything else runs on wavefronts 1-3 in the
 // shadow of those chains, one panel behind, with a workgroup barrier after
 // each panel:
 //   * waves 1 and 2: L21 of the finished panel (rows 64..127 of its 16
@@ -427,6 +427,13 @@ __device__ __forceinline__ void pre_tiles(float* Ls, const __bf16* Ph, int li, i
 template <bool FROM_A, bool LAST, bool PRE>
 __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, float* __restrict__ Wb, DiagLds& sm,
                                            int bk0, int bfirst, bool from_a_rt = false, bool opaque = false) {
+  // Where wave 0's panels switch from right- to left-looking updates
+  // (factor64_step): D1 with PRE from its second panel (the other
+  // wavefronts update D21 and D2 during the first); D1 without PRE from the
+  // first (they are idle then); D2 right-looking, except in the last block,
+  // where wave 0 factors it alone.
+  constexpr int D1LF = PRE ? 1 : 0;
+  constexpr int D2LF = LAST ? 0 : 4;
   float(*const Ts)[16 * TP] = sm.Ts;
   float* const Ls = sm.Ls;
   float* const Ws = sm.Ws;
@@ -570,7 +577,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
       l21_updates<Q, 1>(Ls, Xq, d2, li, lg);         \
   }
   if (wave == 0) {
-    factor64_step<16, P, 0>(Ls, 0, lane, bad1);
+    factor64_step<16, P, 0, D1LF>(Ls, 0, lane, bad1);
   } else if constexpr (PRE) {  // rows 64..127: 26 tiles
     if (wave == 1)
       pre_tiles<4, 0, 5, 5, 0, 4>(Ls, Ph, li, lg);
@@ -581,7 +588,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
   }
   __syncthreads();
   if (wave == 0) {
-    factor64_step<16, P, 1>(Ls, 0, lane, bad1);
+    factor64_step<16, P, 1, D1LF>(Ls, 0, lane, bad1);
   } else if (wave < 3) {
     CHOL_L21_ROUND(0)
   } else if constexpr (!LAST) {
@@ -589,7 +596,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
   }
   __syncthreads();
   if (wave == 0) {
-    factor64_step<16, P, 2>(Ls, 0, lane, bad1);
+    factor64_step<16, P, 2, D1LF>(Ls, 0, lane, bad1);
   } else if (wave < 3) {
     CHOL_L21_ROUND(1)
   } else if constexpr (!LAST) {
@@ -597,7 +604,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
   }
   __syncthreads();
   if (wave == 0) {
-    factor64_step<16, P, 3>(Ls, 0, lane, bad1);
+    factor64_step<16, P, 3, D1LF>(Ls, 0, lane, bad1);
   } else if (wave < 3) {
     CHOL_L21_ROUND(2)
   } else if constexpr (!LAST) {
@@ -683,7 +690,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
   //         (see w21_tiles) in registers of waves 1-2. ----
   floatx4 U[2];
   if (wave == 0) {
-    factor64_step<16, P, 0>(Ls, H, lane, bad2);
+    factor64_step<16, P, 0, D2LF>(Ls, H, lane, bad2);
   } else if (wave == 1) {
     put_x(3);
     wave_lds_sync();
@@ -707,7 +714,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
   }
   if constexpr (!LAST) __syncthreads();
   if (wave == 0) {
-    factor64_step<16, P, 1>(Ls, H, lane, bad2);
+    factor64_step<16, P, 1, D2LF>(Ls, H, lane, bad2);
   } else if constexpr (!LAST) {
     if (wave == 1)
       t1_tiles<0, 3>(Ls, Ws, li, lg);  // column tiles {0, 3} and {1, 2}:
@@ -718,7 +725,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
   }
   if constexpr (!LAST) __syncthreads();
   if (wave == 0) {
-    factor64_step<16, P, 2>(Ls, H, lane, bad2);
+    factor64_step<16, P, 2, D2LF>(Ls, H, lane, bad2);
   } else if constexpr (!LAST) {
     if (wave == 1)
       w21_tiles<0, 0, 2>(Ls, Ws, U, li, lg);
@@ -729,7 +736,7 @@ __device__ __forceinline__ void diag_block(const Batch& p, int64_t b, int k0, fl
   }
   if constexpr (!LAST) __syncthreads();
   if (wave == 0) {
-    factor64_step<16, P, 3>(Ls, H, lane, bad2);
+    factor64_step<16, P, 3, D2LF>(Ls, H, lane, bad2);
   } else if constexpr (!LAST) {
     if (wave == 1)
       w21_tiles<1, 0, 2>(Ls, Ws, U, li, lg);
