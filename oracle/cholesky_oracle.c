@@ -21,7 +21,8 @@
  *
  * Also here: the FP64 reconstruction residual the reference's checker uses
  * (reference/reference.py:170-185, L1 matrix norm of L L^T - A) together with
- * the Frobenius residual of the north-star metric, and a triangular inverse.
+ * the relative Frobenius residual ||A - L L^T||_F / ||A||_F, and a triangular
+ * inverse.
  */
 #include <math.h>
 #include <stdint.h>

@@ -267,7 +267,7 @@ def test_correction_step_planted_full_diagonal_is_first_order():
 
 
 def test_one_correction_step_leaves_fp8_factor_above_target(oracle_mod):
-    # The north star's FP8 form: per-128-tile E4M3 trailing products, then
+    # The FP8 large-n form: per-128-tile E4M3 trailing products, then
     # one FP64 correction step.  At the large-n workload's input class
     # (G G^T/n + 0.1 I, cond ~40) the E4M3 factor sits near 1e-2 and one
     # step brings it only to ~5e-4, 40x above the 1e-5 residual target; two
