@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libchol_amd.so")
 
 MODE_PLAIN = 0
 MODE_MIXED = 1
+MODE_FP8 = 2
 
 E_ARG = -1
 E_WORKSPACE = -2
@@ -41,6 +42,7 @@ SIGNATURES = {
     "chol_plan_potrf": (ctypes.c_int, [_i64, _i64, _i64, _i32, _i32, _p, _i64, _p]),
     "chol_route_counters": (_i64, [_p, _i64]),
     "chol_potrf_f32_noalloc": (ctypes.c_int, [_p, _p, _i64, _i64, _i64, _i64, _i32, _p, _p]),
+    "chol_trailing_update_f32": (ctypes.c_int, [_p, _p, _i64, _i64, _i64, _i64, _i64, _i32, _p]),
 }
 
 

@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 
-_MODES = {"plain": _lib.MODE_PLAIN, "mixed": _lib.MODE_MIXED}
+_MODES = {"plain": _lib.MODE_PLAIN, "mixed": _lib.MODE_MIXED, "fp8": _lib.MODE_FP8}
 
 try:  # raw hipStream_t of the current stream, without building a Stream object
     _raw_stream = torch._C._cuda_getCurrentRawStream
@@ -58,8 +58,12 @@ def cholesky(A: torch.Tensor, mode: str = "plain", return_info: bool = False,
     mode: ``"plain"`` (all-FP32) or ``"mixed"`` (BF16x3 split products in the
     panel solves and trailing updates of the blocked path, which runs for
     n > 256 and for 64 < n <= 256 where the tuning table picks it; the
-    register and LDS kernels are the same in both forms).
-    fallback (mixed form): BF16x3 products carry ~2^-16 relative error, so on
+    register and LDS kernels are the same in both forms), or ``"fp8"``: the
+    mixed form with the bulk trailing updates of orders >= 16384 in FP8 E4M3
+    (rows scaled by powers of two from A's diagonal), the reference's MXFP8
+    large-n form; its residual (~1e-3..1e-2) passes the reference's gate
+    but not the 1e-5 of the mixed form.
+    fallback (mixed and fp8 forms): BF16x3 products carry ~2^-16 relative error, so on
     ill-conditioned input (cond ~1e5 and beyond, e.g. a low-rank matrix plus
     1e-4 I) a pivot can turn non-positive.  With fallback=True the call reads
     info back (one host synchronisation) and refactors those matrices in the
@@ -72,8 +76,8 @@ def cholesky(A: torch.Tensor, mode: str = "plain", return_info: bool = False,
     if m is None:
         raise ValueError(f"mode must be one of {sorted(_MODES)}, got {mode!r}")
     _validate(A)
-    if fallback and m == _lib.MODE_MIXED:
-        return _mixed_with_fallback(A, return_info, out)
+    if fallback and m != _lib.MODE_PLAIN:
+        return _mixed_with_fallback(A, return_info, out, mode)
     lib = _lib.load()
     shape = A.shape
     n = shape[-1]
@@ -107,9 +111,9 @@ def cholesky(A: torch.Tensor, mode: str = "plain", return_info: bool = False,
     return L
 
 
-def _mixed_with_fallback(A, return_info, out):
+def _mixed_with_fallback(A, return_info, out, mode="mixed"):
     global MIXED_FALLBACKS
-    L, info = cholesky(A, mode="mixed", return_info=True, out=out)
+    L, info = cholesky(A, mode=mode, return_info=True, out=out)
     bad = info.reshape(-1) != 0
     if bad.any().item():
         n = A.shape[-1]

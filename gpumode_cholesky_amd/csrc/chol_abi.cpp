@@ -393,11 +393,16 @@ int potrf_dispatch(const char* who, const float* A, float* L, int64_t batch, int
     set_error("%s: batch (%lld) and n (%lld) must be >= 0", who, (long long)batch, (long long)n);
     return CHOL_E_ARG;
   }
-  if (mode != CHOL_MODE_PLAIN && mode != CHOL_MODE_MIXED) {
+  if (mode != CHOL_MODE_PLAIN && mode != CHOL_MODE_MIXED && mode != CHOL_MODE_FP8) {
     set_error("%s: unknown mode %d", who, (int)mode);
     return CHOL_E_MODE;
   }
   if (batch == 0 || n == 0) return 0;
+  // CHOL_MODE_FP8 is the mixed form (BF16x3 panels and narrow updates, its
+  // tuning rows) with E4M3 bulk updates from kF8MinN on.
+  const bool split = mode != CHOL_MODE_PLAIN;
+  const bool fp8 = mode == CHOL_MODE_FP8;
+  if (fp8) mode = CHOL_MODE_MIXED;
   if (n > 65536 || batch > (int64_t)1 << 31) {
     set_error("%s: n=%lld or batch=%lld out of range", who, (long long)n, (long long)batch);
     return CHOL_E_ARG;
@@ -417,6 +422,7 @@ int potrf_dispatch(const char* who, const float* A, float* L, int64_t batch, int
     return CHOL_E_WORKSPACE;
   }
   Batch p{A, L, batch, (int)n, batch > 1 ? batch_stride : n * ld, ld, info};
+  p.f8 = fp8 && n >= kF8MinN;
   if (n <= 64) {
     int rc = launch_small(p, s);
     if (!rc) note_call(RC_SMALL);
@@ -433,7 +439,7 @@ int potrf_dispatch(const char* who, const float* A, float* L, int64_t batch, int
     if (rc) return rc;
     Batch q = p;
     q.first = true;  // A and L share ld: step 0 reads A where it first touches
-    if ((rc = launch_fused(q, reinterpret_cast<float*>(ws), mode == CHOL_MODE_MIXED, s))) return rc;
+    if ((rc = launch_fused(q, reinterpret_cast<float*>(ws), split, s))) return rc;
     note_call(RC_FUSED);
     return 0;
   }
@@ -449,7 +455,7 @@ int potrf_dispatch(const char* who, const float* A, float* L, int64_t batch, int
     q.A = A + b0 * p.bs;
     q.L = L + b0 * p.bs;
     q.info = info ? info + b0 : nullptr;
-    if ((rc = blocked_potrf(q, reinterpret_cast<float*>(ws), mode == CHOL_MODE_MIXED, (int)nbo, s, cus, side_ok)))
+    if ((rc = blocked_potrf(q, reinterpret_cast<float*>(ws), split, (int)nbo, s, cus, side_ok)))
       return rc;
   }
   note_call(RC_BLOCKED);
@@ -485,8 +491,28 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n, int64_t b
                         !capturing);
 }
 
+int chol_trailing_update_f32(const float* A, float* L, int64_t n, int64_t ld, int64_t col0, int64_t k0, int64_t kl,
+                             int32_t mode, void* stream) {
+  clear_error();
+  if (mode != CHOL_MODE_PLAIN && mode != CHOL_MODE_MIXED && mode != CHOL_MODE_FP8) {
+    set_error("chol_trailing_update_f32: unknown mode %d", (int)mode);
+    return CHOL_E_MODE;
+  }
+  if (!A || !L || n <= 0 || n > 65536 || ld < n || col0 < 0 || col0 >= n || k0 < 0 || kl <= 0 || kl % 32 != 0 ||
+      k0 + kl > col0) {
+    set_error("chol_trailing_update_f32: bad arguments (n=%lld ld=%lld col0=%lld k0=%lld kl=%lld)", (long long)n,
+              (long long)ld, (long long)col0, (long long)k0, (long long)kl);
+    return CHOL_E_ARG;
+  }
+  Batch p{A, L, 1, (int)n, n * ld, ld, nullptr};
+  p.f8 = mode == CHOL_MODE_FP8;
+  return launch_syrk256(p, (int)col0, -1, (int)k0, (int)kl, mode != CHOL_MODE_PLAIN, vec_ok(p),
+                        reinterpret_cast<hipStream_t>(stream));
+}
+
 int chol_route(int64_t batch, int64_t n, int32_t mode, int64_t* nbo_out) {
   int64_t nbo = 0;
+  if (mode == CHOL_MODE_FP8) mode = CHOL_MODE_MIXED;  // the mixed form's routes and rows
   int route = CHOL_ROUTE_NONE;
   if (batch > 0 && n > 0) {
     if (n <= 64) {
@@ -526,7 +552,7 @@ int chol_plan_potrf(int64_t batch, int64_t n, int64_t ld, int32_t mode, int32_t 
   const int64_t b = batch < 65535 ? batch : 65535;
   Batch p{fa, fl, b, (int)n, n * ld, ld, nullptr};
   Plan plan;
-  int rc = blocked_potrf(p, nullptr, mode == CHOL_MODE_MIXED, (int)nbo, nullptr, cus > 0 ? cus : 256, true, &plan);
+  int rc = blocked_potrf(p, nullptr, mode != CHOL_MODE_PLAIN, (int)nbo, nullptr, cus > 0 ? cus : 256, true, &plan);
   if (rc) return rc;
   *count = (int64_t)plan.rec.size();
   for (int64_t i = 0; i < *count && i < cap; ++i)

@@ -50,6 +50,9 @@ struct Batch {
   // tril(A) (k_copy_head); step 0's trailing update reads the old values of
   // the columns beyond them from A, and the kernel zeros the upper triangle.
   bool trailing_from_a = false;
+  // CHOL_MODE_FP8 at n >= kF8MinN: the 256-tile bulk updates take E4M3
+  // operands (k_syrk256_e4m3) where the mixed form takes BF16x3.
+  bool f8 = false;
 };
 
 // One matrix of a Batch, dense or ragged.
@@ -328,6 +331,8 @@ int backward(const float* L, const float* Lbar, float* Abar, int64_t batch, int 
 
 // Diagonal block width of the blocked path.
 constexpr int kNB = 128;
+// Smallest order whose bulk updates run in E4M3 in CHOL_MODE_FP8.
+constexpr int kF8MinN = 16384;
 
 // The library's side stream of the current device for drivers other than
 // the dense blocked path: side_begin makes it wait for s's work so far and

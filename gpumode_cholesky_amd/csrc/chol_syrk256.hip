@@ -260,6 +260,177 @@ __global__ __launch_bounds__(NT) void k_syrk256(Batch p, int col0, int col1, int
   }
 }
 
+// E4M3 form (CHOL_MODE_FP8): the same tile, wavefront split and XCD order,
+// with the operands quantised to OCP FP8 E4M3 as they are staged and one
+// v_mfma_f32_16x16x32_fp8_fp8 per 16x16x32 block (three BF16 products in
+// the BF16x3 form).  Scaling is per row, by a power of two taken from A's
+// diagonal: the squares of row r of L sum to A[r][r], so |P[r][k]| <=
+// sqrt(A[r][r]) <= 2^E for every k of every panel, and P[r][k] * 2^(8-E)
+// lies within +-256 (E4M3's largest finite value is 448), with no pass
+// over P to find its maximum.  E4M3 keeps 3 mantissa bits over its whole
+// normal range (2^-6 .. 448), so the row's smaller entries lose no relative
+// precision to the bound.  C -= s_i s_j (Q Q^T)_ij, exact in the scale
+// (powers of two), accumulated in FP32.  Reference counterpart: the MXFP8
+// trailing products of the large-n drivers (submission.py:1626-1675,
+// :3952-3991), which scale per 32-element block.
+constexpr int F8_STAGE = 2 * BT * KC;  // bytes per stage: A rows then B rows (16 KiB)
+
+// 2^E with 2^E >= sqrt(d): d = f 2^e2, f in [0.5, 1), so sqrt(d) < 2^(e2/2)
+// and E = ceil(e2 / 2).  Rows past the matrix (loaded as zeros) take 1.
+__device__ __forceinline__ int row_exp(const Mat& M, int r, int rlim) {
+  if (r >= rlim) return 0;
+  int e2;
+  (void)frexpf(gld(M.A + (int64_t)r * M.lda + r), &e2);
+  return (e2 + 1) >> 1;
+}
+
+// Byte offset of 4-element group q (k = 4q..4q+3) of row r in a stage:
+// 32 bytes per row, 8-byte slot g = q >> 1 at (g ^ ((r >> 3) & 3)), so the
+// 16 rows a fragment read takes hit distinct banks.
+__device__ __forceinline__ int f8_off(int r, int q) {
+  return r * KC + ((((q >> 1) ^ (r >> 3)) & 3) << 3) + ((q & 1) << 2);
+}
+
+// inv: the rows' inverse scales 2^(8-E) in LDS (row r of the operand at inv[r]).
+__device__ __forceinline__ void put_e4m3(unsigned char* __restrict__ s, const float4 (&reg)[PER],
+                                         const float* inv, int tid) {
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int c = tid + u * NT;
+    const float4 v = reg[u];
+    const float f = inv[c >> 3];
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(v.x * f, v.y * f, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(v.z * f, v.w * f, w, true);
+    *reinterpret_cast<int*>(s + f8_off(c >> 3, c & 7)) = w;
+  }
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(NT) void k_syrk256_e4m3(Batch p, int col0, int col1, int k0, int kl) {
+  __shared__ __align__(16) unsigned char f8[2 * F8_STAGE];  // 32 KiB
+  __shared__ float sc[2][BT];                               // row scales 2^(E-8): A rows, B rows
+  __shared__ float si8[2][BT];                              // their inverses 2^(8-E)
+
+  const int64_t b = blockIdx.y;
+  const Mat M = mat_at(p, b);
+  const int c1 = (col1 < 0 || col1 > M.n) ? M.n : col1;
+  if (c1 <= col0) return;
+  const int TI = (M.n - col0 + BT - 1) / BT;
+  const int TJ = (c1 - col0 + BT - 1) / BT;
+  const int T = (int)((int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2);
+  const int bid = blockIdx.x;
+  if (bid >= T) return;
+  int ti, tj;
+  decode_tile_grouped(xcd_tile(bid, T), TI, TJ, 4, ti, tj);
+  const int i0 = col0 + BT * ti;
+  const int j0 = col0 + BT * tj;
+  float* L = M.L;
+  const float* Cs = p.first ? M.A : M.L;
+  const int64_t ld = M.ld;
+  const int64_t ldcs = p.first ? M.lda : M.ld;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave >> 2) * WM;
+  const int wn = (wave & 3) * WN;
+  const int li = lane & 15;
+  const int lg = lane >> 4;
+
+  {
+    // thread t < 256: row i0 + t of the A side; else row j0 + t - 256 of B
+    const int side = tid >= BT, r = tid - side * BT;
+    const int e = side ? row_exp(M, j0 + r, c1) : row_exp(M, i0 + r, M.n);
+    sc[side][r] = ldexpf(1.f, e - 8);
+    si8[side][r] = ldexpf(1.f, 8 - e);
+  }
+  __syncthreads();
+
+  floatx4 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int c = 0; c < FN; ++c) acc[a][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  float4 ra[PER], rb[PER];
+  auto stage = [&](int buf) {
+    unsigned char* s = f8 + buf * F8_STAGE;
+    put_e4m3(s, ra, si8[0], tid);
+    put_e4m3(s + BT * KC, rb, si8[1], tid);
+  };
+
+  load_chunk<VEC>(ra, L, ld, i0, M.n, k0, tid);
+  load_chunk<VEC>(rb, L, ld, j0, c1, k0, tid);
+  stage(0);
+  __syncthreads();
+
+  const int nch = kl / KC;
+  for (int ch = 0; ch < nch; ++ch) {
+    const int cur = ch & 1;
+    const bool more = ch + 1 < nch;
+    if (more) {
+      load_chunk<VEC>(ra, L, ld, i0, M.n, k0 + (ch + 1) * KC, tid);
+      load_chunk<VEC>(rb, L, ld, j0, c1, k0 + (ch + 1) * KC, tid);
+    }
+    const unsigned char* s = f8 + cur * F8_STAGE;
+    const unsigned char* Bq = s + BT * KC;
+    long bq[FN];
+#pragma unroll
+    for (int c = 0; c < FN; ++c) {
+      const int r = wn + c * 16 + li;
+      bq[c] = *reinterpret_cast<const long*>(Bq + f8_off(r, 2 * lg));
+    }
+#pragma unroll
+    for (int a = 0; a < FM; ++a) {
+      // the next chunk's quantisation and LDS stores halfway through this
+      // chunk's products (its buffer is not read in this chunk)
+      if (a == FM / 2 && more) stage(cur ^ 1);
+      const int r = wm + a * 16 + li;
+      const long aq = *reinterpret_cast<const long*>(s + f8_off(r, 2 * lg));
+#pragma unroll
+      for (int c = 0; c < FN; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(aq, bq[c], acc[a][c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  // C -= s_i s_j D on the lower part, as in k_syrk256.
+  const int jl = j0 + wn + li;
+  float sj[FN];
+#pragma unroll
+  for (int c = 0; c < FN; ++c) sj[c] = sc[1][wn + c * 16 + li];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    float old[FM / 2][FN][4];
+#pragma unroll
+    for (int a = 0; a < FM / 2; ++a)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wm + (h * (FM / 2) + a) * 16 + 4 * lg + v;
+        const float* row = Cs + (int64_t)(i < M.n ? i : i0) * ldcs;
+#pragma unroll
+        for (int c = 0; c < FN; ++c) {
+          const int j = jl + c * 16;
+          const bool m = i < M.n && j < c1 && j <= i;
+          old[a][c][v] = gld(m ? row + j : Cs + (int64_t)i0 * ldcs + j0);
+        }
+      }
+#pragma unroll
+    for (int a = 0; a < FM / 2; ++a)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int il = wm + (h * (FM / 2) + a) * 16 + 4 * lg + v;
+        const int i = i0 + il;
+        const float si = sc[0][il];
+        float* row = L + (int64_t)i * ld;
+#pragma unroll
+        for (int c = 0; c < FN; ++c) {
+          const int j = jl + c * 16;
+          if (i < M.n && j < c1 && j <= i) gst(row + j, old[a][c][v] - acc[h * (FM / 2) + a][c][v] * (si * sj[c]));
+        }
+      }
+  }
+}
+
 }  // namespace
 
 int64_t syrk256_tiles(const Batch& p, int col0, int col1) {
@@ -273,6 +444,13 @@ int launch_syrk256(const Batch& p, int col0, int col1, int k0, int kl, bool spli
   const int TJ = ((col1 < 0 ? p.n : col1) - col0 + BT - 1) / BT;
   const int64_t tiles = (int64_t)TJ * TI - (int64_t)TJ * (TJ - 1) / 2;
   dim3 grid((unsigned)tiles, (unsigned)p.batch);
+  if (split && p.f8) {
+    if (vec)
+      hipLaunchKernelGGL((k_syrk256_e4m3<true>), grid, dim3(NT), 0, s, p, col0, col1, k0, kl);
+    else
+      hipLaunchKernelGGL((k_syrk256_e4m3<false>), grid, dim3(NT), 0, s, p, col0, col1, k0, kl);
+    return launch_status("k_syrk256_e4m3");
+  }
   if (split) {
     if (vec)
       hipLaunchKernelGGL((k_syrk256<true, true>), grid, dim3(NT), 0, s, p, col0, col1, k0, kl);

@@ -43,6 +43,12 @@ extern "C" {
 /* Precision modes. */
 #define CHOL_MODE_PLAIN 0 /* all-FP32 arithmetic                            */
 #define CHOL_MODE_MIXED 1 /* BF16x3 split-product panels and trailing updates, FP32 acc */
+/* CHOL_MODE_MIXED with the 256-tile bulk trailing updates of orders >= 16384
+ * in OCP FP8 E4M3 (rows scaled by a power of two from A's diagonal), FP32
+ * accumulation: the reference's MXFP8 large-n products (submission.py:
+ * 1626-1675, :3952-4094); residual ~1e-3..1e-2, inside the reference's
+ * gate, not the 1e-5 of the mixed form.  Below 16384 it is CHOL_MODE_MIXED. */
+#define CHOL_MODE_FP8 2
 
 /* Routes of chol_potrf_f32 (chol_route). */
 #define CHOL_ROUTE_NONE 0    /* batch == 0 or n == 0: nothing launched        */
@@ -77,7 +83,7 @@ size_t chol_workspace_bytes(int64_t batch, int64_t n, int32_t mode);
  *   batch, n      batch >= 0, n >= 0 (batch == 0 or n == 0 launches nothing)
  *   batch_stride  >= n*ld when batch > 1
  *   ld            >= max(n, 1)
- *   mode          CHOL_MODE_PLAIN or CHOL_MODE_MIXED
+ *   mode          CHOL_MODE_PLAIN, CHOL_MODE_MIXED or CHOL_MODE_FP8
  *   info          nullable device int32[batch]
  *   ws, ws_bytes  device workspace of at least chol_workspace_bytes()
  *   stream        hipStream_t to launch on
@@ -103,6 +109,23 @@ int chol_potrf_f32(const float* A, float* L, int64_t batch, int64_t n,
 int chol_potrf_f32_noalloc(const float* A, float* L, int64_t batch, int64_t n,
                            int64_t batch_stride, int64_t ld, int32_t mode,
                            int32_t* info, void* stream);
+
+/*
+ * One outer block's bulk trailing update on its own, by the 256-tile kernel
+ * chol_potrf_f32 runs for it: for one matrix (A and L at row stride ld),
+ * with P = columns [k0, k0 + kl) of L,
+ *     L[i][j] -= sum_k P[i][k] P[j][k]    for col0 <= j <= i < n,
+ * L read and written at those elements only.  mode picks the products:
+ * CHOL_MODE_PLAIN FP32, CHOL_MODE_MIXED BF16x3, CHOL_MODE_FP8 E4M3 with
+ * row r scaled by 2^(8-E), 2^E >= sqrt(A[r][r]) (A supplies only its
+ * diagonal; for any n).  kl a positive multiple of 32, k0 + kl <= col0.
+ * Replaces the trailing SYRK-as-GEMM of the reference's blocked drivers
+ * (submission.py:5163-5184, MXFP8 form :3952-3991), exposed so that the
+ * product can be checked against the oracle on its own.
+ */
+int chol_trailing_update_f32(const float* A, float* L, int64_t n, int64_t ld,
+                             int64_t col0, int64_t k0, int64_t kl, int32_t mode,
+                             void* stream);
 
 /*
  * The route chol_potrf_f32 takes for (batch, n, mode) under the current
