@@ -304,6 +304,27 @@ def extras(dev):
         del a, gL
     out["backward"] = bw
     print("# extras: backward done", file=sys.stderr, flush=True)
+
+    # The fp8 form (E4M3 bulk updates from n = 16384 on) against the mixed
+    # form on the two largest shapes: time per call (median of 3, caches
+    # flushed), the reference's gate and the FP64 residual.
+    f8 = {}
+    for b, n in ((1, 16384), (1, 32768)):
+        a = harness.config_input(b, n, 4700 + n, device=dev)
+        row = {}
+        for mode in ("mixed", "fp8"):
+            row[f"{mode}_us"] = round(_event_us(lambda: cholesky(a, mode=mode), 3), 1)
+            L = cholesky(a, mode=mode)
+            ok, _ = harness.check_implementation(a, L)
+            row[f"{mode}_gate"] = bool(ok)
+            row[f"{mode}_fp64_resid"] = harness.fp64_residual(a, L)
+            del L
+            torch.cuda.empty_cache()
+        f8[f"{b}x{n}"] = row
+        del a
+        torch.cuda.empty_cache()
+    out["fp8_form"] = f8
+    print("# extras: fp8 form done", file=sys.stderr, flush=True)
     return out
 
 
