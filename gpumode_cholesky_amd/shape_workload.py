@@ -6,7 +6,7 @@ on the device; the kernels' time does not depend on the values), or with
 for 64 <= n <= 1024, built through a batched QR).
 
     python -m gpumode_cholesky_amd.shape_workload --batch 2000 --n 300 \
-        [--mode mixed|plain|both] [--calls K] [--route default|fused|blocked|lds] [--nbo W] [--input dense|config]
+        [--mode mixed|plain|fp8|both] [--calls K] [--route default|fused|blocked|lds] [--nbo W] [--input dense|config]
 
 --route forces the route through the tuning table for this shape only:
 fused (the one-workgroup-per-matrix kernel), blocked (outer blocks of
@@ -29,7 +29,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--batch", type=int, required=True)
     ap.add_argument("--n", type=int, required=True)
-    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "both"])
+    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "fp8", "both"])
     ap.add_argument("--calls", type=int, default=3)
     ap.add_argument("--route", default="default", choices=["default", "fused", "blocked", "lds"])
     ap.add_argument("--nbo", type=int, default=256)
