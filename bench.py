@@ -12,13 +12,16 @@ of cache and times, with HIP events, one call per rotated input
 per-call mean over the K steps is that shape's time.  The metric is the
 geometric mean of the 15 per-shape means (microseconds, lower is better).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|mixed|both]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|mixed|fp8|both]
 
 Both forms are timed by default.  The headline value is the mixed form
 (``cholesky(A, mode="mixed")``: BF16x3 panel and trailing products), the
 form the metric's mixed-vs-FP32 speedup is about; the drop-in
 ``custom_kernel`` runs the plain all-FP32 form, whose geomean is reported
 beside it (``geomean_us.plain``), with each form's FP64 residual.
+``--mode fp8`` times the fp8 form alone (E4M3 bulk updates from n = 16384
+on, residual ~1e-2; the extras compare it with the mixed form on the two
+largest shapes in every default run).
 
     python bench.py --paired     # A-B-B-A plain vs mixed per shape (one JSON line)
 """
@@ -47,7 +50,7 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "both"])
+    ap.add_argument("--mode", default="both", choices=["plain", "mixed", "fp8", "both"])
     ap.add_argument("--shapes", default="", help="comma list like 4096x32,1x4096 (default: all 15)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--paired", action="store_true",
@@ -428,7 +431,8 @@ def main():
             for m in modes:
                 row[m]["fwd_vs_torch_cpu"] = fwd[(row["batch"], row["n"])].get(m)
     out = {
-        "metric": "geomean us over 15 batch x n SPD shapes (mixed form; plain in geomean_us)",
+        "metric": ("geomean us over 15 batch x n SPD shapes (mixed form; plain in geomean_us)" if head == "mixed"
+                   else f"geomean us over 15 batch x n SPD shapes ({head} form)"),
         "value": round(geo[head], 2),
         "unit": "us",
         "n_gpus": 1,
@@ -438,7 +442,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if head == "plain" else "f32+bf16x3",
+        "dtype": {"plain": "f32", "mixed": "f32+bf16x3", "fp8": "f32+bf16x3+e4m3"}[head],
         "data": "synthetic",
         "config": {"workload": "15-shape batched Cholesky grid (4096x32 ... 1x32768), FP32 in/out",
                    "form": head, "value_form": f"cholesky(A, mode='{head}')",
